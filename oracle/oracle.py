@@ -1,0 +1,165 @@
+"""CPU oracle for the BLS hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+import this module, and only as the checker (or the timed CPU baseline).
+The product (astro_transit_gpu_amd/) never imports it.
+
+* :func:`power` restates astropy ``BoxLeastSquares.power`` (astropy
+  timeseries/periodograms/bls/core.py:234-330 input preparation,
+  702-720 result assembly) around the C restatement of the compiled
+  ``bls_impl`` in ``bls_oracle.c`` (built by ``oracle/Makefile``).
+* :func:`bls_slow` restates astropy's brute-force ``bls_slow``
+  (bls/methods.py:98-143) in numpy for tiny inputs.  It is an independent
+  definition of every output field, so fast == slow is an invariant check.
+
+Pinning (every function here has at least one):
+  - ``power``/``bls_impl``: bit-for-bit against the outputs of astropy's own
+    bls_impl stored in tests/golden/*.npz, including astropy's
+    test_32bit_bug known answer (test_bls.py:382-397).
+  - ``bls_slow``: astropy's test_fast_method invariant (test_bls.py:414-428,
+    fast == slow on all fields) on the astropy ``data`` fixture, plus the
+    closed forms loglike = W_in*depth^2/2 and snr = depth/depth_err.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libatg_oracle.so")
+FIELDS = ("power", "depth", "depth_err", "duration", "transit_time",
+          "depth_snr", "log_likelihood")
+
+_lib = None
+
+
+def build():
+    """Compile the C restatement with the committed Makefile."""
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle library not built: run make -C oracle")
+        lib = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.atg_oracle_bls.restype = ctypes.c_int
+        lib.atg_oracle_bls.argtypes = [
+            ctypes.c_int64, dp, dp, dp, ctypes.c_int64, dp, ctypes.c_int64,
+            dp, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [dp] * 7
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def bls_impl(t_rel, y_centered, ivar, periods, durations, oversample=10,
+             use_likelihood=True, n_threads=0):
+    """The restated compiled core: returns the 7 output arrays with the
+    transit *phase* (t_ref not yet added)."""
+    lib = _load()
+    t_rel = np.ascontiguousarray(t_rel, dtype=np.float64)
+    y_centered = np.ascontiguousarray(y_centered, dtype=np.float64)
+    ivar = np.ascontiguousarray(ivar, dtype=np.float64)
+    periods = np.ascontiguousarray(np.atleast_1d(periods), dtype=np.float64)
+    durations = np.ascontiguousarray(np.atleast_1d(durations),
+                                     dtype=np.float64)
+    outs = [np.empty(periods.size) for _ in FIELDS]
+    rc = lib.atg_oracle_bls(t_rel.size, _p(t_rel), _p(y_centered), _p(ivar),
+                            periods.size, _p(periods), durations.size,
+                            _p(durations), int(oversample),
+                            int(bool(use_likelihood)), int(n_threads),
+                            *[_p(o) for o in outs])
+    if rc == 1:
+        raise ValueError("invalid period")
+    if rc == 2:
+        raise ValueError("invalid duration")
+    if rc != 0:
+        raise RuntimeError("oracle failed with code %d" % rc)
+    return dict(zip(FIELDS, outs))
+
+
+def prepare(t, y, dy):
+    """astropy power() input preparation (core.py:304-311, 327):
+    FP64 t, t_ref = min(t), ivar = 1/dy^2 (ones if dy is None),
+    y - median(y)."""
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    t_ref = np.min(t)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    if dy is None:
+        ivar = np.ones_like(y)
+    else:
+        ivar = 1.0 / np.ascontiguousarray(dy, dtype=np.float64) ** 2
+    return t - t_ref, y - np.median(y), ivar, t_ref
+
+
+def power(t, y, dy, periods, durations, objective="likelihood",
+          oversample=10, n_threads=0):
+    """astropy BoxLeastSquares(t, y, dy).power(periods, durations,
+    objective=objective, oversample=oversample) on plain float arrays."""
+    periods = np.atleast_1d(np.abs(np.asarray(periods, dtype=np.float64)))
+    durations = np.atleast_1d(np.asarray(durations, dtype=np.float64))
+    if not np.min(periods) > np.max(durations):
+        raise ValueError("The maximum transit duration must be shorter "
+                         "than the minimum period")
+    t_rel, yc, ivar, t_ref = prepare(t, y, dy)
+    out = bls_impl(t_rel, yc, ivar, periods, durations, oversample,
+                   objective == "likelihood", n_threads)
+    out["transit_time"] = out["transit_time"] + t_ref
+    return out
+
+
+def bls_slow(t, y, dy, periods, durations, objective="likelihood",
+             oversample=10):
+    """Brute-force periodogram restated from astropy bls/methods.py:98-143
+    (``_bls_slow_one``), with the power() preparation applied first."""
+    t_rel, y, ivar, t_ref = prepare(t, y, dy)
+    use_likelihood = objective == "likelihood"
+    durations = np.atleast_1d(np.asarray(durations, dtype=np.float64))
+    rows = []
+    min_t = np.min(t_rel)
+    for period in np.atleast_1d(periods):
+        best = (-np.inf, None)
+        hp = 0.5 * period
+        for dur in durations:
+            d_phase = dur / oversample
+            for t0 in np.arange(0, period + d_phase, d_phase):
+                m_in = np.abs((t_rel - min_t - t0 + hp) % period - hp) \
+                    < 0.5 * dur
+                m_out = ~m_in
+                ivar_in = np.sum(ivar[m_in])
+                ivar_out = np.sum(ivar[m_out])
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    y_in = np.sum(y[m_in] * ivar[m_in]) / ivar_in
+                    y_out = np.sum(y[m_out] * ivar[m_out]) / ivar_out
+                    depth = y_out - y_in
+                    depth_err = np.sqrt(1.0 / ivar_in + 1.0 / ivar_out)
+                    snr = depth / depth_err
+                loglike = -0.5 * np.sum((y_in - y[m_in]) ** 2 * ivar[m_in])
+                loglike += 0.5 * np.sum((y_out - y[m_in]) ** 2 * ivar[m_in])
+                objective_v = loglike if use_likelihood else snr
+                if depth > 0 and objective_v > best[0]:
+                    best = (objective_v,
+                            (objective_v, depth, depth_err, dur,
+                             (t0 + min_t) % period, snr, loglike))
+        rows.append(best[1])
+    out = dict(zip(FIELDS, map(np.array, zip(*rows))))
+    out["transit_time"] = out["transit_time"] + t_ref
+    return out
+
+
+def load_golden(path):
+    """Load a golden fixture (data only; no pickles)."""
+    d = np.load(path, allow_pickle=False)
+    case = dict(t=d["t"], y=d["y"], dy=d["dy"] if "dy" in d.files else None,
+                periods=d["periods"], durations=d["durations"],
+                oversample=int(d["oversample"]),
+                objective=str(d["objective"]))
+    case["out"] = {k: d["out_" + k] for k in FIELDS}
+    return case
