@@ -1,0 +1,1530 @@
+// atg_bls.hip -- Box Least Squares periodogram for MI355X (gfx950, wave64).
+//
+// Pipeline per call (all on one HIP stream):
+//   k_prep     one workgroup per light curve: t_ref = min(t), NaN / dy
+//              checks, grid checks, exact median of y (radix select on the
+//              order-preserving key of the FP64 value), fixed-point scales,
+//              then the "lane-major" point workspace: t' = t - t_ref (FP64),
+//              round(ivar * 2^sw) and round((y - med) * ivar * 2^sy) as int64.
+//   k_items    work-item offsets (ceil(periods / periods_per_item) per curve)
+//              and the reset of the persistent queue counter.
+//   k_main     persistent workgroups pull (curve, period tile) items from an
+//              atomic counter.  For up to 8 periods at once each lane walks
+//              its contiguous run of points, folds them in FP64 with the
+//              oracle's exact bin rule, merges consecutive points of the same
+//              bin in registers and adds each run to the LDS histogram with
+//              one 64-bit integer atomic (order-independent => bit-exact).
+//              Then per period: padding copy, int64 prefix scan, conversion
+//              to FP64, sliding-box scan over every duration, argmax with the
+//              oracle's first-wins tie order, and the 7 output fields.
+//   k_final    per-curve argmax of power and status.
+//
+// Semantics follow astropy BoxLeastSquares.power(method="fast") (the
+// reference's oracle, src/astrotransit_gpu/search/cpu_reference_bls.py:
+// 11-16); the binning mechanics are those the reference transcribes in
+// src/astrotransit_gpu/search/kernels/vbls_v42_parity.cu:39-129.
+//
+// Compiled with -ffp-contract=off: every multiply and add that mirrors the
+// oracle is rounded separately; fused multiply-adds appear only where fma()
+// is written.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "atg_bls.h"
+
+namespace {
+
+constexpr int NT = 256;      // threads per workgroup = 4 wave64
+constexpr int NW = NT / 64;  // waves per workgroup
+constexpr int SUBMAX = 8;    // periods folded together in one pass
+constexpr int DMAX = 64;     // durations per light curve
+
+struct LcParams {
+    double t_ref;     // min(t)
+    double y_med;     // median(y)
+    double bd;        // bin duration = min(durations) / oversample
+    double w_unit;    // 2^-sw: value of one weight quantum
+    double y_unit;    // 2^-sy: value of one weight*flux quantum
+    double W_tot;     // total weight (real units, from the quantized sum)
+    double Y_tot;     // total weight*flux
+    long long Wq_tot; // quantized totals
+    long long Yq_tot;
+    long long ws_off; // first slot of the curve in the point workspace
+    int L;            // rows of the lane-major layout = ceil(n / NT)
+    int n;            // points
+    int status;       // ATG_LC_*
+    int n_dur;
+};
+
+struct Batch {
+    long long n_lc;
+    const double *t;
+    const void *y;
+    const void *dy;
+    int y_dt, dy_dt;
+    const long long *lc_off;
+    const double *periods;
+    const long long *per_off;
+    const double *durs;
+    const long long *dur_off;
+    int oversample, objective, mode;
+};
+
+struct Out {
+    double *power, *depth, *depth_err, *duration, *transit_time, *depth_snr,
+        *log_like;
+    long long *best_index;
+    int *status;
+};
+
+struct Work {
+    LcParams *prm;
+    int *lc_flags;        // per-curve flags raised by k_main (LDS overflow)
+    long long *item_off;  // [n_lc + 1]
+    unsigned int *counter;
+    double *ws_t;         // lane-major t'
+    long long *ws_w;      // lane-major quantized ivar
+    long long *ws_y;      // lane-major quantized ivar * (y - median)
+    int *ncand;           // [n_lc] periods refined in the oracle's order
+    long long *cand;      // [n_lc * CAP] their global period indices
+};
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ double load_val(const void *p, int dt,
+                                           long long i) {
+    return dt == ATG_F32 ? (double)((const float *)p)[i]
+                         : ((const double *)p)[i];
+}
+
+// Order-preserving unsigned key of an FP64 value (non-NaN).
+__device__ __forceinline__ unsigned long long okey(double v) {
+    unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double ikey(unsigned long long k) {
+    unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double((long long)u);
+}
+
+template <typename T, typename Op>
+__device__ T block_reduce(T v, Op op, T *scratch) {
+    for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o));
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) scratch[w] = v;
+    __syncthreads();
+    T r = scratch[0];
+#pragma unroll
+    for (int i = 1; i < NW; ++i) r = op(r, scratch[i]);
+    return r;
+}
+
+// Exclusive scan over the workgroup in thread order; *total gets the sum.
+template <typename T>
+__device__ T block_excl_scan(T v, T *scratch, T *total) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    T inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        T u = __shfl_up(inc, o);
+        if (l >= o) inc += u;
+    }
+    __syncthreads();
+    if (l == 63) scratch[w] = inc;
+    __syncthreads();
+    T off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        T s = scratch[i];
+        if (i < w) off += s;
+        tot += s;
+    }
+    *total = tot;
+    return off + inc - v;
+}
+
+struct MinOp {
+    __device__ double operator()(double a, double b) const { return fmin(a, b); }
+};
+struct MaxOp {
+    __device__ double operator()(double a, double b) const { return fmax(a, b); }
+};
+struct OrOp {
+    __device__ int operator()(int a, int b) const { return a | b; }
+};
+template <typename T>
+struct AddOp {
+    __device__ T operator()(T a, T b) const { return a + b; }
+};
+
+// Bin index of relative time t (>= 0) for one period, equal to the oracle's
+// (int)((t - P*floor(t/P)) / bd) + 1.  The two divisions are replaced by
+// products with the rounded reciprocals; a product can differ from the
+// correctly rounded quotient by a few ulp, which changes floor() only when
+// the quotient is within 2^-50 (relative) of an integer, and exactly then
+// the true division is taken.
+__device__ __forceinline__ int fold_bin(double t, double P, double invP,
+                                        double bd, double inv_bd) {
+    double r = t * invP;
+    double q = floor(r);
+    double fr = r - q;
+    double tol = r * 0x1p-50;
+    if (fr <= tol || fr >= 1.0 - tol) q = floor(t / P);
+    double ph = t - P * q;
+    double x = ph * inv_bd;
+    double xi = trunc(x);
+    double fx = x - xi;
+    double tolx = fabs(x) * 0x1p-50;
+    if (fx <= tolx || fx >= 1.0 - tolx) xi = trunc(ph / bd);
+    return (int)xi + 1;
+}
+
+// ------------------------------------------------------------------ k_prep
+
+// Two order statistics k0 <= k1 of the curve's y values (as FP64) by an
+// 8-pass radix select over the order-preserving 64-bit key.
+__device__ void select2(const void *y, int dt, long long off, int n, int k0,
+                        int k1, double *v0, double *v1, unsigned int *hist,
+                        int *sel, unsigned int *uscratch) {
+    unsigned long long pre0 = 0, pre1 = 0, mask = 0;
+    int kk0 = k0, kk1 = k1;
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = 56 - 8 * pass;
+        for (int i = threadIdx.x; i < 512; i += NT) hist[i] = 0;
+        __syncthreads();
+        unsigned c0 = 0, c1 = 0, d0 = 0, d1 = 0;
+        for (int i = threadIdx.x; i < n; i += NT) {
+            unsigned long long key = okey(load_val(y, dt, off + i));
+            unsigned d = (unsigned)(key >> shift) & 255u;
+            if ((key & mask) == pre0) {
+                if (c0 && d != d0) { atomicAdd(&hist[d0], c0); c0 = 0; }
+                d0 = d;
+                ++c0;
+            }
+            if ((key & mask) == pre1) {
+                if (c1 && d != d1) { atomicAdd(&hist[256 + d1], c1); c1 = 0; }
+                d1 = d;
+                ++c1;
+            }
+        }
+        if (c0) atomicAdd(&hist[d0], c0);
+        if (c1) atomicAdd(&hist[256 + d1], c1);
+        __syncthreads();
+        unsigned h0 = hist[threadIdx.x], h1 = hist[256 + threadIdx.x], tot;
+        unsigned e0 = block_excl_scan(h0, uscratch, &tot);
+        unsigned e1 = block_excl_scan(h1, uscratch, &tot);
+        if (e0 <= (unsigned)kk0 && (unsigned)kk0 < e0 + h0) {
+            sel[0] = threadIdx.x;
+            sel[1] = kk0 - (int)e0;
+        }
+        if (e1 <= (unsigned)kk1 && (unsigned)kk1 < e1 + h1) {
+            sel[2] = threadIdx.x;
+            sel[3] = kk1 - (int)e1;
+        }
+        __syncthreads();
+        pre0 |= (unsigned long long)sel[0] << shift;
+        kk0 = sel[1];
+        pre1 |= (unsigned long long)sel[2] << shift;
+        kk1 = sel[3];
+        mask |= 0xffull << shift;
+        __syncthreads();
+    }
+    *v0 = ikey(pre0);
+    *v1 = ikey(pre1);
+}
+
+__global__ __launch_bounds__(NT) void k_prep(Batch b, Work w) {
+    __shared__ unsigned int hist[512];
+    __shared__ int sel[4];
+    __shared__ unsigned int uscr[NW];
+    __shared__ double dscr[NW];
+    __shared__ int iscr[NW];
+    __shared__ long long lscr[NW];
+
+    const int lc = blockIdx.x;
+    const long long off = b.lc_off[lc];
+    const long long n64 = b.lc_off[lc + 1] - off;
+    const long long po = b.per_off[lc], np = b.per_off[lc + 1] - po;
+    const long long dof = b.dur_off[lc], nd = b.dur_off[lc + 1] - dof;
+    const int n = (int)n64;
+    const int tid = threadIdx.x;
+    if (tid == 0) w.lc_flags[lc] = 0;  // workspace bytes are reused across shapes
+    LcParams P;
+    memset(&P, 0, sizeof(P));
+    P.n = n;
+    P.n_dur = (int)nd;
+    P.ws_off = off + (long long)lc * NT;
+    P.L = (n + NT - 1) / NT;
+    int status = ATG_LC_OK;
+    if (n <= 0 || np <= 0 || nd <= 0) status = ATG_LC_EMPTY;
+    else if (nd > DMAX) status = ATG_LC_TOO_MANY_DURATIONS;
+
+    // grid checks in the oracle's order: periods, then durations
+    double pmin = INFINITY, dmin = INFINITY, dmax = -INFINITY;
+    int bad_grid = 0;
+    for (long long i = tid; i < np; i += NT) {
+        double p = b.periods[po + i];
+        if (!(p >= DBL_EPSILON)) bad_grid |= 1;
+        pmin = fmin(pmin, p);
+    }
+    for (long long i = tid; i < nd; i += NT) {
+        double d = b.durs[dof + i];
+        if (!(d >= DBL_EPSILON) || isnan(d)) bad_grid |= 2;
+        dmin = fmin(dmin, d);
+        dmax = fmax(dmax, d);
+    }
+    pmin = block_reduce(pmin, MinOp(), dscr);
+    dmin = block_reduce(dmin, MinOp(), dscr);
+    dmax = block_reduce(dmax, MaxOp(), dscr);
+    bad_grid = block_reduce(bad_grid, OrOp(), iscr);
+    if (status == ATG_LC_OK) {
+        if (bad_grid & 1) status = ATG_LC_BAD_PERIOD;
+        else if ((bad_grid & 2) || dmax > pmin) status = ATG_LC_BAD_DURATION;
+    }
+
+    // pass A: min(t), NaN and dy checks
+    double tmin = INFINITY;
+    int bad = 0;
+    for (int i = tid; i < n; i += NT) {
+        double tv = b.t[off + i];
+        double yv = load_val(b.y, b.y_dt, off + i);
+        if (isnan(tv) || isnan(yv)) bad |= 1;
+        tmin = fmin(tmin, tv);
+        if (b.dy) {
+            double d = load_val(b.dy, b.dy_dt, off + i);
+            if (!(d > 0.0)) bad |= 2;
+        }
+    }
+    tmin = block_reduce(tmin, MinOp(), dscr);
+    bad = block_reduce(bad, OrOp(), iscr);
+    if (status == ATG_LC_OK) {
+        if (bad & 1) status = ATG_LC_NAN;
+        else if (bad & 2) status = ATG_LC_BAD_DY;
+    }
+    P.t_ref = tmin;
+    P.status = status;
+    if (status != ATG_LC_OK) {
+        if (tid == 0) w.prm[lc] = P;
+        return;
+    }
+
+    // exact median (numpy: mean of the two middle values for even n)
+    double v0, v1;
+    select2(b.y, b.y_dt, off, n, (n - 1) / 2, n / 2, &v0, &v1, hist, sel,
+            uscr);
+    const double med = (n & 1) ? v0 : (v0 + v1) * 0.5;
+    P.y_med = med;
+
+    // pass B: magnitudes for the fixed-point scales
+    double sw_sum = 0.0, sy_sum = 0.0;
+    for (int i = tid; i < n; i += NT) {
+        double iv = 1.0;
+        if (b.dy) {
+            double d = load_val(b.dy, b.dy_dt, off + i);
+            iv = 1.0 / (d * d);
+        }
+        double yc = load_val(b.y, b.y_dt, off + i) - med;
+        sw_sum += iv;
+        sy_sum += fabs(yc * iv);
+    }
+    sw_sum = block_reduce(sw_sum, AddOp<double>(), dscr);
+    sy_sum = block_reduce(sy_sum, AddOp<double>(), dscr);
+    // scale so that the sum of |quantized| values stays below 2^60; the
+    // padding copies at most double a prefix sum, well inside int64.
+    int sw = 59 - ilogb(sw_sum);
+    int sy = (sy_sum > 0.0 && isfinite(sy_sum)) ? 59 - ilogb(sy_sum) : 0;
+    sw = max(-1000, min(1000, sw));
+    sy = max(-1000, min(1000, sy));
+    const double wscale = ldexp(1.0, sw), yscale = ldexp(1.0, sy);
+    P.w_unit = ldexp(1.0, -sw);
+    P.y_unit = ldexp(1.0, -sy);
+    P.bd = dmin / (double)b.oversample;
+
+    // pass C: lane-major workspace; lane j owns points [j*L, (j+1)*L)
+    long long wq_sum = 0, yq_sum = 0;
+    const int L = P.L;
+    for (int i = 0; i < L; ++i) {
+        int src = tid * L + i;
+        int s = src < n ? src : n - 1;
+        double tp = b.t[off + s] - tmin;
+        long long wq = 0, yq = 0;
+        if (src < n) {
+            double iv = 1.0;
+            if (b.dy) {
+                double d = load_val(b.dy, b.dy_dt, off + s);
+                iv = 1.0 / (d * d);
+            }
+            double yc = load_val(b.y, b.y_dt, off + s) - med;
+            wq = llrint(iv * wscale);
+            yq = llrint((yc * iv) * yscale);
+        }
+        long long slot = P.ws_off + (long long)i * NT + tid;
+        w.ws_t[slot] = tp;
+        w.ws_w[slot] = wq;
+        w.ws_y[slot] = yq;
+        wq_sum += wq;
+        yq_sum += yq;
+    }
+    wq_sum = block_reduce(wq_sum, AddOp<long long>(), lscr);
+    yq_sum = block_reduce(yq_sum, AddOp<long long>(), lscr);
+    P.Wq_tot = wq_sum;
+    P.Yq_tot = yq_sum;
+    P.W_tot = (double)wq_sum * P.w_unit;
+    P.Y_tot = (double)yq_sum * P.y_unit;
+    if (tid == 0) w.prm[lc] = P;
+}
+
+// ----------------------------------------------------------------- k_items
+
+__global__ __launch_bounds__(NT) void k_items(long long n_lc,
+                                              const long long *per_off,
+                                              int per_item, Work w) {
+    __shared__ long long lscr[NW];
+    long long carry = 0;
+    for (long long base = 0; base < n_lc; base += NT) {
+        long long i = base + threadIdx.x;
+        long long c = 0;
+        if (i < n_lc) c = (per_off[i + 1] - per_off[i] + per_item - 1) / per_item;
+        long long tot;
+        long long e = block_excl_scan(c, lscr, &tot);
+        if (i < n_lc) w.item_off[i] = carry + e;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        w.item_off[n_lc] = carry;
+        *w.counter = 0u;
+    }
+}
+
+// ------------------------------------------------------------------ k_main
+
+struct Best {
+    double s;
+    int idx;
+};
+
+__device__ __forceinline__ bool better(double s, int i, double bs, int bi) {
+    return s > bs || (s == bs && i < bi);
+}
+
+__device__ __forceinline__ void write_nan(const Out &o, long long p) {
+    const double nan = __longlong_as_double(0x7ff8000000000000ll);
+    o.power[p] = nan;
+    o.depth[p] = nan;
+    o.depth_err[p] = nan;
+    o.duration[p] = nan;
+    o.transit_time[p] = nan;
+    o.depth_snr[p] = nan;
+    o.log_like[p] = nan;
+}
+
+// Padding, prefix scan and box scan of one period whose int64 histogram
+// occupies bins [base, base + nb] of binW / binY; writes period p.
+__device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
+                              long long p, double period, int nb, int base,
+                              long long *binW, long long *binY,
+                              const int *s_m, int nd, long long *lscr,
+                              double *dscr, int *iscr) {
+    const int tid = threadIdx.x;
+    const int os = b.oversample;
+    // padding: bins[nb-os+j-1] = bins[j], j = 1..os (an assignment)
+    if (nb - os > os) {
+        for (int j = 1 + tid; j <= os; j += NT) {
+            binW[base + nb - os + j - 1] = binW[base + j];
+            binY[base + nb - os + j - 1] = binY[base + j];
+        }
+    } else if (tid == 0) {  // overlapping ranges: the oracle's order
+        for (int j = 1, d = nb - os; j <= os; ++j, ++d) {
+            binW[base + d] = binW[base + j];
+            binY[base + d] = binY[base + j];
+        }
+    }
+    __syncthreads();
+
+    // inclusive prefix over bins 0..nb in int64 (exact), then FP64
+    const int nval = nb + 1;
+    const int seg = (nval + NT - 1) / NT;
+    const int b0 = min(tid * seg, nval), b1 = min(b0 + seg, nval);
+    long long sw = 0, sy = 0;
+    for (int i = b0; i < b1; ++i) {
+        sw += binW[base + i];
+        sy += binY[base + i];
+    }
+    long long totw, toty;
+    long long ow = block_excl_scan(sw, lscr, &totw);
+    long long oy = block_excl_scan(sy, lscr, &toty);
+    double *Cw = (double *)binW;
+    double *Cy = (double *)binY;
+    for (int i = b0; i < b1; ++i) {
+        ow += binW[base + i];
+        oy += binY[base + i];
+        Cw[base + i] = (double)ow * P.w_unit;
+        Cy[base + i] = (double)oy * P.y_unit;
+    }
+    __syncthreads();
+
+    // box scan: start n, duration k (m bins), window (n, n+m]
+    const double Wt = P.W_tot, Yt = P.Y_tot;
+    const bool like = b.objective == ATG_OBJ_LIKELIHOOD;
+    double bs = -1.0;
+    int bi = INT_MAX;
+    for (int s = tid; s < nb; s += NT) {
+        const double w0 = Cw[base + s], y0 = Cy[base + s];
+        for (int k = 0; k < nd; ++k) {
+            const int m = s_m[k];
+            if (s + m > nb) continue;
+            const double win = Cw[base + s + m] - w0;
+            const double yin = Cy[base + s + m] - y0;
+            const double wout = Wt - win;
+            if (win < DBL_EPSILON || wout < DBL_EPSILON) continue;
+            // depth * W_in * W_out = Y_tot*W_in - Y_in*W_tot
+            const double num = fma(Yt, win, -(yin * Wt));
+            if (num < 0.0) continue;
+            const double den = like ? win * wout * wout : win * wout;
+            const double sc = (num * num) / den;
+            const int idx = k * nval + s;
+            if (better(sc, idx, bs, bi)) {
+                bs = sc;
+                bi = idx;
+            }
+        }
+    }
+    // workgroup argmax (highest score, then lowest (k, n))
+    for (int o = 32; o > 0; o >>= 1) {
+        double os_ = __shfl_xor(bs, o);
+        int oi = __shfl_xor(bi, o);
+        if (better(os_, oi, bs, bi)) {
+            bs = os_;
+            bi = oi;
+        }
+    }
+    const int wv = tid >> 6, l = tid & 63;
+    __syncthreads();
+    if (l == 0) {
+        dscr[wv] = bs;
+        iscr[wv] = bi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int i = 1; i < NW; ++i)
+            if (better(dscr[i], iscr[i], bs, bi)) {
+                bs = dscr[i];
+                bi = iscr[i];
+            }
+        if (bi == INT_MAX) {  // no window qualifies: power -inf as astropy
+            o.power[p] = -INFINITY;
+            o.depth[p] = 0.0;
+            o.depth_err[p] = 0.0;
+            o.duration[p] = 0.0;
+            o.transit_time[p] = P.t_ref;
+            o.depth_snr[p] = 0.0;
+            o.log_like[p] = 0.0;
+        } else {
+            const int k = bi / nval, s = bi - k * nval, m = s_m[k];
+            const double win = Cw[base + s + m] - Cw[base + s];
+            const double yin_s = Cy[base + s + m] - Cy[base + s];
+            const double wout = Wt - win;
+            const double y_in = yin_s / win;
+            const double y_out = (Yt - yin_s) / wout;
+            const double depth = y_out - y_in;
+            const double derr = sqrt(1.0 / win + 1.0 / wout);
+            const double snr = depth / derr;
+            const double ll = 0.5 * win * depth * depth;
+            const double dur = m * P.bd;
+            const double phase = fmod(s * P.bd + 0.5 * dur + 0.0, period);
+            o.power[p] = like ? ll : snr;
+            o.depth[p] = depth;
+            o.depth_err[p] = derr;
+            o.duration[p] = dur;
+            o.transit_time[p] = phase + P.t_ref;
+            o.depth_snr[p] = snr;
+            o.log_like[p] = ll;
+        }
+    }
+    __syncthreads();
+}
+
+// Destinations of a point in raw bin r once the padding assignment
+// bins[nb-os+j-1] = bins[j] (j = 1..os) has been applied: its own bin unless
+// that bin is overwritten, plus the copy of bins 1..os.  Valid when the copy
+// range does not overlap its source (nb - os > os).
+__device__ __forceinline__ void pad_targets(int r, int nb, int os, int &j1,
+                                            int &j2) {
+    j1 = (r < nb - os || r == nb) ? r : -1;
+    j2 = (r >= 1 && r <= os) ? r + nb - os - 1 : -1;
+}
+
+__device__ __forceinline__ void seg_add(int j, int s0, int hi, long long wv,
+                                        long long yv, long long &cw,
+                                        long long &cy, long long *binW,
+                                        long long *binY) {
+    if (j < 0) return;
+    if (j < s0) {
+        cw += wv;
+        cy += yv;
+    } else if (j <= hi) {
+        __hip_atomic_fetch_add(&binW[j - s0], wv, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&binY[j - s0], yv, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+// One period whose n_bins + 1 exceeds the LDS histogram: the start
+// positions are processed in segments of S = lds_bins - mmax - 1; each
+// segment bins every point again, keeps bins [s0, s0 + S + mmax] in LDS and
+// carries the (exact, int64) sum of the bins below s0.
+__device__ void big_period(const Batch &b, const Out &o, const Work &w,
+                           const LcParams &P, long long p, int nb,
+                           int lds_bins, long long *binW, long long *binY,
+                           const int *s_m, int nd, int mmax, long long *lscr,
+                           double *dscr, int *iscr, double *dscr2,
+                           double *dscr3) {
+    const int tid = threadIdx.x, os = b.oversample;
+    const double period = b.periods[p], invP = 1.0 / period;
+    const double bd = P.bd, inv_bd = 1.0 / bd;
+    const int S = lds_bins - mmax - 1;
+    const int nval = nb + 1;
+    const bool like = b.objective == ATG_OBJ_LIKELIHOOD;
+    const double Wt = P.W_tot, Yt = P.Y_tot;
+    double bs = -1.0, bwin = 0.0, byin = 0.0;
+    int bi = INT_MAX;
+    const double *tp = w.ws_t + P.ws_off + tid;
+    const long long *wp = w.ws_w + P.ws_off + tid;
+    const long long *yp = w.ws_y + P.ws_off + tid;
+    double *Cw = (double *)binW;
+    double *Cy = (double *)binY;
+    for (int s0 = 0; s0 < nb; s0 += S) {
+        const int hi = min(s0 + S + mmax, nb), len = hi - s0 + 1;
+        for (int i = tid; i < len; i += NT) {
+            binW[i] = 0;
+            binY[i] = 0;
+        }
+        __syncthreads();
+        long long cw = 0, cy = 0;
+        for (int i = 0; i < P.L; ++i) {
+            const long long wv = wp[(long long)i * NT];
+            const long long yv = yp[(long long)i * NT];
+            if (wv == 0 && yv == 0) continue;
+            const double tv = tp[(long long)i * NT];
+            const int r = min(max(fold_bin(tv, period, invP, bd, inv_bd), 0),
+                              nb);
+            int j1, j2;
+            pad_targets(r, nb, os, j1, j2);
+            seg_add(j1, s0, hi, wv, yv, cw, cy, binW, binY);
+            seg_add(j2, s0, hi, wv, yv, cw, cy, binW, binY);
+        }
+        __syncthreads();
+        cw = block_reduce(cw, AddOp<long long>(), lscr);
+        cy = block_reduce(cy, AddOp<long long>(), lscr);
+        const int seg = (len + NT - 1) / NT;
+        const int b0 = min(tid * seg, len), b1 = min(b0 + seg, len);
+        long long sw = 0, sy = 0;
+        for (int i = b0; i < b1; ++i) {
+            sw += binW[i];
+            sy += binY[i];
+        }
+        long long tw, ty;
+        long long ow = block_excl_scan(sw, lscr, &tw) + cw;
+        long long oy = block_excl_scan(sy, lscr, &ty) + cy;
+        for (int i = b0; i < b1; ++i) {
+            ow += binW[i];
+            oy += binY[i];
+            Cw[i] = (double)ow * P.w_unit;
+            Cy[i] = (double)oy * P.y_unit;
+        }
+        __syncthreads();
+        const int nend = min(s0 + S, nb);
+        for (int n = s0 + tid; n < nend; n += NT) {
+            const int l = n - s0;
+            const double w0 = Cw[l], y0 = Cy[l];
+            for (int k = 0; k < nd; ++k) {
+                const int m = s_m[k];
+                if (n + m > nb) continue;
+                const double win = Cw[l + m] - w0;
+                const double yin = Cy[l + m] - y0;
+                const double wout = Wt - win;
+                if (win < DBL_EPSILON || wout < DBL_EPSILON) continue;
+                const double num = fma(Yt, win, -(yin * Wt));
+                if (num < 0.0) continue;
+                const double den = like ? win * wout * wout : win * wout;
+                const double sc = (num * num) / den;
+                const int idx = k * nval + n;
+                if (better(sc, idx, bs, bi)) {
+                    bs = sc;
+                    bi = idx;
+                    bwin = win;
+                    byin = yin;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int ofs = 32; ofs > 0; ofs >>= 1) {
+        const double os_ = __shfl_xor(bs, ofs);
+        const int oi = __shfl_xor(bi, ofs);
+        const double ow_ = __shfl_xor(bwin, ofs), oy_ = __shfl_xor(byin, ofs);
+        if (better(os_, oi, bs, bi)) {
+            bs = os_;
+            bi = oi;
+            bwin = ow_;
+            byin = oy_;
+        }
+    }
+    const int wv = tid >> 6, l = tid & 63;
+    __syncthreads();
+    if (l == 0) {
+        dscr[wv] = bs;
+        iscr[wv] = bi;
+        dscr2[wv] = bwin;
+        dscr3[wv] = byin;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int i = 1; i < NW; ++i)
+            if (better(dscr[i], iscr[i], bs, bi)) {
+                bs = dscr[i];
+                bi = iscr[i];
+                bwin = dscr2[i];
+                byin = dscr3[i];
+            }
+        if (bi == INT_MAX) {
+            o.power[p] = -INFINITY;
+            o.depth[p] = 0.0;
+            o.depth_err[p] = 0.0;
+            o.duration[p] = 0.0;
+            o.transit_time[p] = P.t_ref;
+            o.depth_snr[p] = 0.0;
+            o.log_like[p] = 0.0;
+        } else {
+            const int k = bi / nval, n = bi - k * nval, m = s_m[k];
+            const double wout = Wt - bwin;
+            const double y_in = byin / bwin;
+            const double y_out = (Yt - byin) / wout;
+            const double depth = y_out - y_in;
+            const double derr = sqrt(1.0 / bwin + 1.0 / wout);
+            const double snr = depth / derr;
+            const double ll = 0.5 * bwin * depth * depth;
+            const double dur = m * P.bd;
+            o.power[p] = like ? ll : snr;
+            o.depth[p] = depth;
+            o.depth_err[p] = derr;
+            o.duration[p] = dur;
+            o.transit_time[p] = fmod(n * P.bd + 0.5 * dur + 0.0, period) +
+                                P.t_ref;
+            o.depth_snr[p] = snr;
+            o.log_like[p] = ll;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void flush_run(long long *binW, long long *binY,
+                                          int at, long long aw,
+                                          long long ay) {
+    if (aw != 0 || ay != 0) {
+        __hip_atomic_fetch_add(&binW[at], aw, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&binY[at], ay, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_main(Batch b, Out o, Work w,
+                                             int per_item, int lds_bins) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    long long *binW = (long long *)smem;
+    long long *binY = binW + lds_bins;
+    __shared__ int s_m[DMAX];
+    __shared__ unsigned int s_item;
+    __shared__ long long lscr[NW];
+    __shared__ double dscr[NW], dscr2[NW], dscr3[NW];
+    __shared__ int iscr[NW];
+
+    const int tid = threadIdx.x;
+    const long long n_lc = b.n_lc;
+    const unsigned int total = (unsigned int)w.item_off[n_lc];
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) s_item = atomicAdd(w.counter, 1u);
+        __syncthreads();
+        const unsigned int item = s_item;
+        if (item >= total) break;
+        // curve of the item: last lc with item_off[lc] <= item
+        long long lo = 0, hi = n_lc;
+        while (hi - lo > 1) {
+            long long mid = (lo + hi) >> 1;
+            if ((unsigned long long)w.item_off[mid] <= item) lo = mid;
+            else hi = mid;
+        }
+        while (lo + 1 < n_lc && (unsigned long long)w.item_off[lo + 1] <= item)
+            ++lo;
+        const int lc = (int)lo;
+        const LcParams P = w.prm[lc];
+        const long long pbeg = b.per_off[lc] +
+                               (long long)(item - w.item_off[lc]) * per_item;
+        const long long pend = min(pbeg + per_item, b.per_off[lc + 1]);
+        if (P.status != ATG_LC_OK) {
+            for (long long p = pbeg + tid; p < pend; p += NT) write_nan(o, p);
+            continue;
+        }
+        const int nd = P.n_dur;
+        const long long dof = b.dur_off[lc];
+        if (tid < nd) s_m[tid] = (int)round(b.durs[dof + tid] / P.bd);
+        __syncthreads();
+        int mmax = 0;
+        for (int k = 0; k < nd; ++k) mmax = max(mmax, s_m[k]);
+        const double bd = P.bd;
+        const double inv_bd = 1.0 / bd;
+        const int os = b.oversample;
+
+        long long p = pbeg;
+        while (p < pend) {
+            // greedy sub-batch of periods that fit in the LDS histogram
+            double per[SUBMAX], inv[SUBMAX];
+            int nb[SUBMAX], base[SUBMAX];
+            int cnt = 0, used = 0;
+#pragma unroll
+            for (int s = 0; s < SUBMAX; ++s) {
+                per[s] = 1.0;
+                inv[s] = 1.0;
+                nb[s] = 0;
+                base[s] = 0;
+            }
+#pragma unroll
+            for (int s = 0; s < SUBMAX; ++s) {
+                if (cnt == s && p + s < pend) {
+                    const double ps = b.periods[p + s];
+                    const int nbs = (int)ceil(ps / bd) + os;
+                    if (used + nbs + 1 <= lds_bins) {
+                        per[s] = ps;
+                        inv[s] = 1.0 / ps;
+                        nb[s] = nbs;
+                        base[s] = used;
+                        used += nbs + 1;
+                        ++cnt;
+                    }
+                }
+            }
+            if (cnt == 0) {  // this period alone exceeds the LDS histogram
+                const int nbp = (int)ceil(b.periods[p] / bd) + os;
+                if (nbp - os > os && lds_bins - mmax - 1 >= 1) {
+                    big_period(b, o, w, P, p, nbp, lds_bins, binW, binY, s_m,
+                               nd, mmax, lscr, dscr, iscr, dscr2, dscr3);
+                } else if (tid == 0) {
+                    write_nan(o, p);
+                    atomicOr(&w.lc_flags[lc], 1);
+                }
+                ++p;
+                continue;
+            }
+            for (int i = tid; i < used; i += NT) {
+                binW[i] = 0;
+                binY[i] = 0;
+            }
+            __syncthreads();
+
+            // fold + run-merged binning over the lane's points
+            int cur[SUBMAX];
+            long long aw[SUBMAX], ay[SUBMAX];
+#pragma unroll
+            for (int s = 0; s < SUBMAX; ++s) {
+                cur[s] = -1;
+                aw[s] = 0;
+                ay[s] = 0;
+            }
+            const double *tp = w.ws_t + P.ws_off + tid;
+            const long long *wp = w.ws_w + P.ws_off + tid;
+            const long long *yp = w.ws_y + P.ws_off + tid;
+            for (int i = 0; i < P.L; ++i) {
+                const double tv = tp[(long long)i * NT];
+                const long long wv = wp[(long long)i * NT];
+                const long long yv = yp[(long long)i * NT];
+#pragma unroll
+                for (int s = 0; s < SUBMAX; ++s) {
+                    if (s < cnt) {
+                        int ind = fold_bin(tv, per[s], inv[s], bd, inv_bd);
+                        ind = min(max(ind, 0), nb[s]);
+                        if (ind != cur[s]) {
+                            if (cur[s] >= 0)
+                                flush_run(binW, binY, base[s] + cur[s], aw[s],
+                                          ay[s]);
+                            cur[s] = ind;
+                            aw[s] = wv;
+                            ay[s] = yv;
+                        } else {
+                            aw[s] += wv;
+                            ay[s] += yv;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < SUBMAX; ++s)
+                if (s < cnt && cur[s] >= 0)
+                    flush_run(binW, binY, base[s] + cur[s], aw[s], ay[s]);
+            __syncthreads();
+
+            int bse = 0;
+            for (int s = 0; s < cnt; ++s) {
+                const double ps = b.periods[p + s];
+                const int nbs = (int)ceil(ps / bd) + os;
+                finish_period(b, o, P, p + s, ps, nbs, bse, binW, binY, s_m,
+                              nd, lscr, dscr, iscr);
+                bse += nbs + 1;
+            }
+            p += cnt;
+        }
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------- k_cands / k_exact
+//
+// The int64 bins make every window sum exact, so two periods whose best
+// windows hold the same points get exactly equal power.  The oracle sums in
+// FP64 in point order, and its rounding decides such ties (and near-ties
+// below ~1e-13).  In parity mode the periods of each curve whose power is
+// within TIE_REL of the curve's maximum are therefore recomputed with the
+// oracle's own operation order: per-bin FP64 sums in point order, the
+// padding assignment, the sequential cumulative sum and the box scan with
+// the oracle's formulas.  Their outputs then equal the oracle's bit for bit
+// and the argmax follows the oracle's.
+
+namespace {
+
+constexpr int CAP = 16;            // refined periods per curve
+constexpr double TIE_REL = 1e-9;   // >> relative error of the int64 path
+
+__global__ __launch_bounds__(NT) void k_cands(Batch b, Out o, Work w) {
+    __shared__ double dscr[NW];
+    __shared__ int iscr[NW];
+    const int lc = blockIdx.x;
+    const long long po = b.per_off[lc], np = b.per_off[lc + 1] - po;
+    if (b.mode != ATG_MODE_PARITY || w.prm[lc].status != ATG_LC_OK ||
+        w.lc_flags[lc]) {
+        if (threadIdx.x == 0) w.ncand[lc] = 0;
+        return;
+    }
+    double m = -INFINITY;
+    for (long long i = threadIdx.x; i < np; i += NT) {
+        const double v = o.power[po + i];
+        if (v > m) m = v;  // NaN never compares greater
+    }
+    m = block_reduce(m, MaxOp(), dscr);
+    if (!(m > -INFINITY)) {
+        if (threadIdx.x == 0) w.ncand[lc] = 0;
+        return;
+    }
+    const double thr = m - TIE_REL * fabs(m);
+    int count = 0;
+    for (long long base = 0; base < np && count < CAP; base += NT) {
+        const long long i = base + threadIdx.x;
+        const int f = (i < np && o.power[po + i] >= thr) ? 1 : 0;
+        int tot;
+        const int e = block_excl_scan(f, iscr, &tot);
+        if (f && count + e < CAP) w.cand[(long long)lc * CAP + count + e] = po + i;
+        count += tot;
+    }
+    if (threadIdx.x == 0) w.ncand[lc] = min(count, CAP);
+}
+
+__global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
+                                              int lds_bins) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *my = (double *)smem;        // sum of y*ivar per bin
+    double *mi = my + lds_bins;         // sum of ivar per bin
+    __shared__ int s_ind[64];
+    __shared__ double s_yv[64], s_iv[64];
+    __shared__ int s_m[DMAX];
+    __shared__ double s_tot[2];
+    const int lc = blockIdx.y, c = blockIdx.x, lane = threadIdx.x;
+    if (c >= w.ncand[lc]) return;
+    const LcParams P = w.prm[lc];
+    const long long p = w.cand[(long long)lc * CAP + c];
+    const double period = b.periods[p];
+    const double bd = P.bd, inv_bd = 1.0 / bd, invP = 1.0 / period;
+    const int os = b.oversample;
+    const int nb = (int)ceil(period / bd) + os;
+    const long long off = b.lc_off[lc];
+    const int n = P.n, nd = P.n_dur;
+    if (lane < nd) s_m[lane] = (int)round(b.durs[b.dur_off[lc] + lane] / bd);
+    __syncthreads();
+    int mmax = 0;
+    for (int k = 0; k < nd; ++k) mmax = max(mmax, s_m[k]);
+    // in LDS: raw bins, then the oracle's sequential padding; otherwise
+    // segments of start positions with the padded destinations of each
+    // point and the exact FP64 prefix carried from segment to segment
+    const bool fits = nb + 1 <= lds_bins;
+    if (!fits && !(nb - os > os && lds_bins - mmax - 1 >= 1)) return;
+    const int S = fits ? nb + 1 : lds_bins - mmax - 1;
+    const bool like = b.objective == ATG_OBJ_LIKELIHOOD;
+    const int nval = nb + 1;
+    double sum_y = 0.0, sum_iv = 0.0, cprev_y = 0.0, cprev_i = 0.0;
+    double bo = -INFINITY, byin = 0.0, bivin = 0.0;
+    int bi = INT_MAX;
+    for (int s0 = 0; s0 < nb; s0 += S) {
+        const int hi = fits ? nb : min(s0 + S + mmax, nb);
+        const int len = hi - s0 + 1;
+        for (int i = lane; i < len; i += 64) {
+            my[i] = 0.0;
+            mi[i] = 0.0;
+        }
+        __syncthreads();
+        double ay = 0.0, ai = 0.0;
+        int cur = -1;
+        for (int base = 0; base < n; base += 64) {
+            const int i = base + lane;
+            if (i < n) {
+                const double tp = b.t[off + i] - P.t_ref;
+                double iv = 1.0;
+                if (b.dy) {
+                    const double d = load_val(b.dy, b.dy_dt, off + i);
+                    iv = 1.0 / (d * d);
+                }
+                const double yc = load_val(b.y, b.y_dt, off + i) - P.y_med;
+                s_yv[lane] = yc * iv;
+                s_iv[lane] = iv;
+                s_ind[lane] = min(max(fold_bin(tp, period, invP, bd, inv_bd),
+                                      0), nb);
+            }
+            __syncthreads();
+            if (lane == 0) {
+                const int cnt = min(64, n - base);
+                for (int j = 0; j < cnt; ++j) {
+                    const double yv = s_yv[j], iv = s_iv[j];
+                    const int ind = s_ind[j];
+                    if (s0 == 0) {
+                        sum_y += yv;
+                        sum_iv += iv;
+                    }
+                    if (fits) {
+                        // the bin's running sum stays in registers while
+                        // consecutive points share it: same order of adds
+                        if (ind != cur) {
+                            if (cur >= 0) {
+                                my[cur] = ay;
+                                mi[cur] = ai;
+                            }
+                            cur = ind;
+                            ay = my[ind];
+                            ai = mi[ind];
+                        }
+                        ay += yv;
+                        ai += iv;
+                    } else {
+                        int j1, j2;
+                        pad_targets(ind, nb, os, j1, j2);
+                        if (j1 >= s0 && j1 <= hi) {
+                            my[j1 - s0] += yv;
+                            mi[j1 - s0] += iv;
+                        }
+                        if (j2 >= s0 && j2 <= hi) {
+                            my[j2 - s0] += yv;
+                            mi[j2 - s0] += iv;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (lane == 0) {
+            if (fits) {
+                if (cur >= 0) {
+                    my[cur] = ay;
+                    mi[cur] = ai;
+                }
+                for (int j = 1, d = nb - os; j <= os; ++j, ++d) {
+                    my[d] = my[j];
+                    mi[d] = mi[j];
+                }
+            }
+            double cy = s0 == 0 ? my[0] : cprev_y + my[0];
+            double ci = s0 == 0 ? mi[0] : cprev_i + mi[0];
+            my[0] = cy;
+            mi[0] = ci;
+            for (int l = 1; l < len; ++l) {
+                cy = cy + my[l];
+                ci = ci + mi[l];
+                my[l] = cy;
+                mi[l] = ci;
+            }
+            if (!fits && s0 + S < nb) {
+                cprev_y = my[S - 1];
+                cprev_i = mi[S - 1];
+            }
+            if (s0 == 0) {
+                s_tot[0] = sum_y;
+                s_tot[1] = sum_iv;
+            }
+        }
+        __syncthreads();
+        const double ty = s_tot[0], ti = s_tot[1];
+        const int nend = min(s0 + S, nb);
+        for (int st = s0 + lane; st < nend; st += 64) {
+            const int l = st - s0;
+            for (int k = 0; k < nd; ++k) {
+                const int m = s_m[k];
+                if (st + m > nb) continue;
+                const double ys = my[l + m] - my[l];
+                const double is = mi[l + m] - mi[l];
+                double y_in = ys, iv_in = is;
+                double y_out = ty - y_in;
+                double iv_out = ti - iv_in;
+                if (iv_in < DBL_EPSILON || iv_out < DBL_EPSILON) continue;
+                y_in /= iv_in;
+                y_out /= iv_out;
+                const double depth = y_out - y_in;
+                const double derr = sqrt(1.0 / iv_in + 1.0 / iv_out);
+                const double snr = depth / derr;
+                const double ll = 0.5 * iv_in * depth * depth;
+                const double obj = like ? ll : snr;
+                const int idx = k * nval + st;
+                if (depth >= 0.0 && better(obj, idx, bo, bi)) {
+                    bo = obj;
+                    bi = idx;
+                    byin = ys;
+                    bivin = is;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int ofs = 32; ofs > 0; ofs >>= 1) {
+        const double ob = __shfl_xor(bo, ofs);
+        const int oi = __shfl_xor(bi, ofs);
+        const double oy = __shfl_xor(byin, ofs), ov = __shfl_xor(bivin, ofs);
+        if (better(ob, oi, bo, bi)) {
+            bo = ob;
+            bi = oi;
+            byin = oy;
+            bivin = ov;
+        }
+    }
+    if (lane == 0 && bi != INT_MAX) {
+        const double ty = s_tot[0], ti = s_tot[1];
+        const int k = bi / nval, st = bi - k * nval, m = s_m[k];
+        double y_in = byin, iv_in = bivin;
+        double y_out = ty - y_in;
+        double iv_out = ti - iv_in;
+        y_in /= iv_in;
+        y_out /= iv_out;
+        const double depth = y_out - y_in;
+        const double derr = sqrt(1.0 / iv_in + 1.0 / iv_out);
+        const double snr = depth / derr;
+        const double ll = 0.5 * iv_in * depth * depth;
+        const double dur = m * bd;
+        o.power[p] = like ? ll : snr;
+        o.depth[p] = depth;
+        o.depth_err[p] = derr;
+        o.duration[p] = dur;
+        o.transit_time[p] = fmod(st * bd + 0.5 * dur + 0.0, period) + P.t_ref;
+        o.depth_snr[p] = snr;
+        o.log_like[p] = ll;
+    }
+}
+
+// ----------------------------------------------------------------- k_final
+
+__global__ __launch_bounds__(NT) void k_final(Batch b, Out o, Work w) {
+    __shared__ double dscr[NW];
+    __shared__ long long lscr[NW];
+    __shared__ int iscr[NW];
+    const int lc = blockIdx.x;
+    const long long po = b.per_off[lc], np = b.per_off[lc + 1] - po;
+    // numpy argmax order: a NaN wins (first NaN), else the first maximum
+    double bv = -INFINITY;
+    long long bi = LLONG_MAX;
+    int bn = 0;
+    for (long long i = threadIdx.x; i < np; i += NT) {
+        const double v = o.power[po + i];
+        const int vn = isnan(v) ? 1 : 0;
+        bool take = vn > bn || (vn == bn && (vn ? i < bi
+                                               : (v > bv || (v == bv && i < bi))));
+        if (take) {
+            bv = v;
+            bi = i;
+            bn = vn;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        double ov = __shfl_xor(bv, off);
+        long long oi = __shfl_xor(bi, off);
+        int on = __shfl_xor(bn, off);
+        bool take = on > bn || (on == bn && (on ? oi < bi
+                                               : (ov > bv || (ov == bv && oi < bi))));
+        if (take) {
+            bv = ov;
+            bi = oi;
+            bn = on;
+        }
+    }
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) {
+        dscr[wv] = bv;
+        lscr[wv] = bi;
+        iscr[wv] = bn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < NW; ++i) {
+            bool take = iscr[i] > bn ||
+                        (iscr[i] == bn &&
+                         (bn ? lscr[i] < bi
+                             : (dscr[i] > bv || (dscr[i] == bv && lscr[i] < bi))));
+            if (take) {
+                bv = dscr[i];
+                bi = lscr[i];
+                bn = iscr[i];
+            }
+        }
+        if (o.best_index) o.best_index[lc] = (bi == LLONG_MAX) ? 0 : bi;
+        if (o.status) {
+            int st = w.prm[lc].status;
+            if (st == ATG_LC_OK && w.lc_flags[lc]) st = ATG_LC_TOO_MANY_BINS;
+            o.status[lc] = st;
+        }
+        w.lc_flags[lc] = 0;
+    }
+}
+
+}  // namespace
+
+// ===================================================================== ABI
+
+struct atg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    std::string err;
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    atg_tuning tune{8, 4096, 2, 0};
+    int num_cu = 0;
+    size_t lds_max = 65536;
+    size_t lds_attr_set = 65536;
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<hipEvent_t> spare;
+    double t_sum = 0.0;
+    int t_cnt = 0;
+};
+
+namespace {
+
+int fail(atg_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(atg_ctx *c, hipError_t e, const char *what) {
+    return fail(c, ATG_ERR_HIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct Layout {
+    size_t prm, flags, items, counter, ncand, cand, t, w, y, total;
+};
+
+Layout layout_of(const atg_bls_batch *b) {
+    Layout L;
+    const size_t nlc = (size_t)b->n_lc;
+    const size_t slots = (size_t)b->n_points + nlc * NT;
+    size_t o = 0;
+    L.prm = o;     o = align256(o + nlc * sizeof(LcParams));
+    L.flags = o;   o = align256(o + nlc * sizeof(int));
+    L.items = o;   o = align256(o + (nlc + 1) * sizeof(long long));
+    L.counter = o; o = align256(o + sizeof(unsigned int));
+    L.ncand = o;   o = align256(o + nlc * sizeof(int));
+    L.cand = o;    o = align256(o + nlc * CAP * sizeof(long long));
+    L.t = o;       o = align256(o + slots * sizeof(double));
+    L.w = o;       o = align256(o + slots * sizeof(long long));
+    L.y = o;       o = align256(o + slots * sizeof(long long));
+    L.total = o;
+    return L;
+}
+
+int check_batch(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o) {
+    if (!b || !o) return fail(c, ATG_ERR_ARG, "null batch or output");
+    if (b->n_lc < 1 || b->n_lc > INT_MAX)
+        return fail(c, ATG_ERR_ARG, "n_lc must be in [1, 2^31)");
+    if (b->n_points < 0 || b->n_periods < 1 || b->n_durations < 1)
+        return fail(c, ATG_ERR_ARG, "empty periods/durations or negative size");
+    if (b->n_points > ((long long)1 << 40))
+        return fail(c, ATG_ERR_ARG, "too many points");
+    if (!b->lc_offsets || !b->periods || !b->period_offsets ||
+        !b->durations || !b->dur_offsets)
+        return fail(c, ATG_ERR_ARG, "null offsets or grid pointer");
+    if (b->n_points > 0 && (!b->t || !b->y))
+        return fail(c, ATG_ERR_ARG, "null t or y");
+    if (b->y_dtype != ATG_F32 && b->y_dtype != ATG_F64)
+        return fail(c, ATG_ERR_ARG, "y_dtype must be ATG_F32 or ATG_F64");
+    if (b->dy && b->dy_dtype != ATG_F32 && b->dy_dtype != ATG_F64)
+        return fail(c, ATG_ERR_ARG, "dy_dtype must be ATG_F32 or ATG_F64");
+    if (b->oversample < 1) return fail(c, ATG_ERR_ARG, "oversample must be >= 1");
+    if (b->objective != ATG_OBJ_LIKELIHOOD && b->objective != ATG_OBJ_SNR)
+        return fail(c, ATG_ERR_ARG, "objective must be likelihood or snr");
+    if (b->mode != ATG_MODE_PARITY && b->mode != ATG_MODE_FAST)
+        return fail(c, ATG_ERR_ARG, "mode must be parity or fast");
+    if (!o->power || !o->depth || !o->depth_err || !o->duration ||
+        !o->transit_time || !o->depth_snr || !o->log_likelihood)
+        return fail(c, ATG_ERR_ARG, "null output array");
+    return ATG_OK;
+}
+
+int ensure_workspace(atg_ctx *c, size_t need) {
+    if (need <= c->ws_bytes) return ATG_OK;
+    size_t want = need + need / 4;
+    if (c->budget && want > c->budget) want = need;
+    if (c->budget && need > c->budget)
+        return fail(c, ATG_ERR_BUDGET,
+                    "workspace of " + std::to_string(need) +
+                        " bytes exceeds the HBM budget of " +
+                        std::to_string(c->budget));
+    if (c->ws) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return hip_fail(c, e, "hipStreamSynchronize");
+        hipFree(c->ws);
+        c->ws = nullptr;
+        c->ws_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&c->ws, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        c->ws = nullptr;
+        return fail(c, ATG_ERR_BUDGET,
+                    "hipMalloc of " + std::to_string(want) + " bytes failed");
+    }
+    c->ws_bytes = want;
+    return ATG_OK;
+}
+
+Batch to_batch(const atg_bls_batch *b) {
+    Batch k;
+    k.n_lc = b->n_lc;
+    k.t = b->t;
+    k.y = b->y;
+    k.dy = b->dy;
+    k.y_dt = b->y_dtype;
+    k.dy_dt = b->dy_dtype;
+    k.lc_off = (const long long *)b->lc_offsets;
+    k.periods = b->periods;
+    k.per_off = (const long long *)b->period_offsets;
+    k.durs = b->durations;
+    k.dur_off = (const long long *)b->dur_offsets;
+    k.oversample = b->oversample;
+    k.objective = b->objective;
+    k.mode = b->mode;
+    return k;
+}
+
+}  // namespace
+
+extern "C" {
+
+int atg_abi_version(void) { return ATG_ABI_VERSION; }
+
+int atg_ctx_create(int device, void *stream, uint64_t hbm_budget_bytes,
+                   atg_ctx **out) {
+    if (!out) return ATG_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+        (void)hipGetLastError();
+        return ATG_ERR_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+        return ATG_ERR_DEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return ATG_ERR_DEVICE;
+    atg_ctx *c = new atg_ctx();
+    c->device = device;
+    c->stream = (hipStream_t)stream;
+    c->budget = hbm_budget_bytes;
+    c->num_cu = prop.multiProcessorCount;
+    c->lds_max = prop.sharedMemPerBlock;
+    *out = c;
+    return ATG_OK;
+}
+
+void atg_ctx_destroy(atg_ctx *c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    else hipDeviceSynchronize();
+    for (auto &pr : c->pending) {
+        hipEventDestroy(pr.first);
+        hipEventDestroy(pr.second);
+    }
+    for (auto e : c->spare) hipEventDestroy(e);
+    if (c->ws) hipFree(c->ws);
+    delete c;
+}
+
+int atg_ctx_set_stream(atg_ctx *c, void *stream) {
+    if (!c) return ATG_ERR_ARG;
+    c->stream = (hipStream_t)stream;
+    return ATG_OK;
+}
+
+const char *atg_last_error(const atg_ctx *c) {
+    return c ? c->err.c_str() : "null context";
+}
+
+int atg_ctx_set_tuning(atg_ctx *c, const atg_tuning *t) {
+    if (!c || !t) return ATG_ERR_ARG;
+    atg_tuning n = c->tune;
+    if (t->periods_per_item) n.periods_per_item = t->periods_per_item;
+    if (t->lds_bins) n.lds_bins = t->lds_bins;
+    if (t->wg_per_cu) n.wg_per_cu = t->wg_per_cu;
+    if (n.periods_per_item < 1 || n.periods_per_item > 64)
+        return fail(c, ATG_ERR_ARG, "periods_per_item must be in [1, 64]");
+    if (n.lds_bins < 64 || (size_t)n.lds_bins * 16 > c->lds_max)
+        return fail(c, ATG_ERR_ARG,
+                    "lds_bins must be >= 64 and fit " +
+                        std::to_string(c->lds_max) + " bytes of LDS");
+    if (n.wg_per_cu < 1 || n.wg_per_cu > 32)
+        return fail(c, ATG_ERR_ARG, "wg_per_cu must be in [1, 32]");
+    c->tune = n;
+    return ATG_OK;
+}
+
+int atg_ctx_get_tuning(const atg_ctx *c, atg_tuning *t) {
+    if (!c || !t) return ATG_ERR_ARG;
+    *t = c->tune;
+    return ATG_OK;
+}
+
+int atg_bls_workspace_size(const atg_ctx *c, const atg_bls_batch *b,
+                           uint64_t *bytes) {
+    (void)c;
+    if (!b || !bytes || b->n_lc < 1 || b->n_points < 0) return ATG_ERR_ARG;
+    *bytes = layout_of(b).total;
+    return ATG_OK;
+}
+
+int atg_bls_run(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o) {
+    if (!c) return ATG_ERR_ARG;
+    int rc = check_batch(c, b, o);
+    if (rc != ATG_OK) return rc;
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+    const Layout L = layout_of(b);
+    rc = ensure_workspace(c, L.total);
+    if (rc != ATG_OK) return rc;
+    char *base = (char *)c->ws;
+    Work w;
+    w.prm = (LcParams *)(base + L.prm);
+    w.lc_flags = (int *)(base + L.flags);
+    w.item_off = (long long *)(base + L.items);
+    w.counter = (unsigned int *)(base + L.counter);
+    w.ws_t = (double *)(base + L.t);
+    w.ws_w = (long long *)(base + L.w);
+    w.ws_y = (long long *)(base + L.y);
+    w.ncand = (int *)(base + L.ncand);
+    w.cand = (long long *)(base + L.cand);
+    Batch kb = to_batch(b);
+    Out ko;
+    ko.power = o->power;
+    ko.depth = o->depth;
+    ko.depth_err = o->depth_err;
+    ko.duration = o->duration;
+    ko.transit_time = o->transit_time;
+    ko.depth_snr = o->depth_snr;
+    ko.log_like = o->log_likelihood;
+    ko.best_index = (long long *)o->best_index;
+    ko.status = o->status;
+
+    const int nlc = (int)b->n_lc;
+    const size_t lds = (size_t)c->tune.lds_bins * 16;
+    if (lds > c->lds_attr_set) {
+        e = hipFuncSetAttribute((const void *)k_main,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+        if (e != hipSuccess) return hip_fail(c, e, "hipFuncSetAttribute");
+        e = hipFuncSetAttribute((const void *)k_exact,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+        if (e != hipSuccess) return hip_fail(c, e, "hipFuncSetAttribute");
+        c->lds_attr_set = lds;
+    }
+    hipLaunchKernelGGL(k_prep, dim3(nlc), dim3(NT), 0, c->stream, kb, w);
+    hipLaunchKernelGGL(k_items, dim3(1), dim3(NT), 0, c->stream,
+                       (long long)b->n_lc, kb.per_off,
+                       c->tune.periods_per_item, w);
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (c->timing) {
+        for (int i = 0; i < 2; ++i) {
+            hipEvent_t ev;
+            if (!c->spare.empty()) {
+                ev = c->spare.back();
+                c->spare.pop_back();
+            } else if (hipEventCreate(&ev) != hipSuccess) {
+                return fail(c, ATG_ERR_HIP, "hipEventCreate");
+            }
+            (i ? ev1 : ev0) = ev;
+        }
+        hipEventRecord(ev0, c->stream);
+    }
+    const int grid = c->num_cu * c->tune.wg_per_cu;
+    hipLaunchKernelGGL(k_main, dim3(grid), dim3(NT), lds, c->stream, kb, ko,
+                       w, c->tune.periods_per_item, c->tune.lds_bins);
+    if (c->timing) {
+        hipEventRecord(ev1, c->stream);
+        c->pending.emplace_back(ev0, ev1);
+    }
+    hipLaunchKernelGGL(k_cands, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w);
+    hipLaunchKernelGGL(k_exact, dim3(CAP, nlc), dim3(64), lds, c->stream, kb,
+                       ko, w, c->tune.lds_bins);
+    hipLaunchKernelGGL(k_final, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(c, e, "kernel launch");
+    return ATG_OK;
+}
+
+int atg_bls_run_sync(atg_ctx *c, const atg_bls_batch *b,
+                     const atg_bls_out *o) {
+    int rc = atg_bls_run(c, b, o);
+    if (rc != ATG_OK) return rc;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamSynchronize");
+    return ATG_OK;
+}
+
+int atg_timing_main_kernel(atg_ctx *c, int enable, int reset,
+                           double *avg_ms) {
+    if (!c) return ATG_ERR_ARG;
+    for (auto &pr : c->pending) {
+        hipEventSynchronize(pr.second);
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+            c->t_sum += ms;
+            c->t_cnt += 1;
+        }
+        c->spare.push_back(pr.first);
+        c->spare.push_back(pr.second);
+    }
+    c->pending.clear();
+    int n = c->t_cnt;
+    if (avg_ms) *avg_ms = n ? c->t_sum / n : 0.0;
+    if (reset) {
+        c->t_sum = 0.0;
+        c->t_cnt = 0;
+    }
+    c->timing = enable != 0;
+    return n;
+}
+
+}  // extern "C"

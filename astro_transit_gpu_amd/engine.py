@@ -1,0 +1,292 @@
+"""Device-side driver of libatg_bls.so over torch-ROCm buffers.
+
+torch is used only to own device memory and streams; every computation on
+the path runs in the HIP kernels of csrc/atg_bls.hip.
+
+* :class:`DeviceBatch`  ragged CSR batch of light curves resident in HBM.
+* :class:`Context`      one library context (device + stream + workspace
+  budget + tiling) and :meth:`Context.run`.
+"""
+import ctypes
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _abi
+
+FIELDS = ("power", "depth", "depth_err", "duration", "transit_time",
+          "depth_snr", "log_likelihood")
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _dev(device):
+    d = torch.device(device if device is not None else "cuda")
+    if d.type != "cuda":
+        raise ValueError("device must be a GPU device, got %s" % d)
+    if d.index is None:
+        d = torch.device("cuda", torch.cuda.current_device())
+    return d
+
+
+@dataclass
+class DeviceBatch:
+    """CSR batch: curve i owns points lc_offsets[i]:lc_offsets[i+1], periods
+    period_offsets[i]:... and durations dur_offsets[i]:...  All tensors live
+    on one GPU; offsets are int64, t/periods/durations FP64, y/dy FP32 or
+    FP64 (dy may be None: unit weights)."""
+    t: torch.Tensor
+    y: torch.Tensor
+    dy: Optional[torch.Tensor]
+    lc_offsets: torch.Tensor
+    periods: torch.Tensor
+    period_offsets: torch.Tensor
+    durations: torch.Tensor
+    dur_offsets: torch.Tensor
+    host_period_offsets: np.ndarray = field(repr=False)
+
+    @property
+    def n_lc(self):
+        return int(self.lc_offsets.numel() - 1)
+
+    @property
+    def n_periods(self):
+        return int(self.periods.numel())
+
+    @staticmethod
+    def from_host(ts: Sequence[np.ndarray], ys: Sequence[np.ndarray],
+                  dys: Optional[Sequence[Optional[np.ndarray]]],
+                  periods, durations, device=None, y_dtype=None):
+        """Pack host arrays into a device batch.  ``periods`` and
+        ``durations`` are either one array shared by every curve or one array
+        per curve.  ``dys`` is None (unit weights for all) or one array per
+        curve.  y/dy keep FP32 if every input is FP32, else FP64."""
+        dev = _dev(device)
+        n = len(ts)
+        if n == 0:
+            raise ValueError("empty batch")
+
+        def per_curve(a):
+            if isinstance(a, (list, tuple)) and len(a) > 0 and \
+                    np.ndim(a[0]) >= 1:
+                if len(a) != n:
+                    raise ValueError("need one grid per light curve")
+                return [np.atleast_1d(np.asarray(x, dtype=np.float64))
+                        for x in a]
+            a = np.atleast_1d(np.asarray(a, dtype=np.float64))
+            return [a] * n
+
+        pers = per_curve(periods)
+        durs = per_curve(durations)
+        if y_dtype is None:
+            all32 = all(np.asarray(y).dtype == np.float32 for y in ys) and (
+                dys is None or all(np.asarray(d).dtype == np.float32
+                                   for d in dys))
+            y_dtype = np.float32 if all32 else np.float64
+        lc_off = np.zeros(n + 1, dtype=np.int64)
+        lc_off[1:] = np.cumsum([np.size(t) for t in ts])
+        p_off = np.zeros(n + 1, dtype=np.int64)
+        p_off[1:] = np.cumsum([p.size for p in pers])
+        d_off = np.zeros(n + 1, dtype=np.int64)
+        d_off[1:] = np.cumsum([d.size for d in durs])
+
+        def cat(arrs, dt):
+            if len(arrs) == 0:
+                return np.zeros(0, dt)
+            return np.ascontiguousarray(np.concatenate(
+                [np.asarray(a, dtype=dt).ravel() for a in arrs]))
+
+        def up(a):
+            return torch.from_numpy(a).pin_memory().to(dev, non_blocking=True)
+
+        b = DeviceBatch(
+            t=up(cat(ts, np.float64)),
+            y=up(cat(ys, y_dtype)),
+            dy=None if dys is None else up(cat(dys, y_dtype)),
+            lc_offsets=up(lc_off),
+            periods=up(cat(pers, np.float64)),
+            period_offsets=up(p_off),
+            durations=up(cat(durs, np.float64)),
+            dur_offsets=up(d_off),
+            host_period_offsets=p_off,
+        )
+        torch.cuda.current_stream(dev).synchronize()
+        return b
+
+
+@dataclass
+class DeviceResult:
+    """Per-period fields (FP64, aligned with the batch periods), per-curve
+    best period index and status."""
+    power: torch.Tensor
+    depth: torch.Tensor
+    depth_err: torch.Tensor
+    duration: torch.Tensor
+    transit_time: torch.Tensor
+    depth_snr: torch.Tensor
+    log_likelihood: torch.Tensor
+    best_index: torch.Tensor
+    status: torch.Tensor
+    period_offsets: np.ndarray
+
+    @staticmethod
+    def empty(n_periods, n_lc, period_offsets, device):
+        f = [torch.empty(n_periods, dtype=torch.float64, device=device)
+             for _ in FIELDS]
+        return DeviceResult(*f,
+                            best_index=torch.empty(n_lc, dtype=torch.int64,
+                                                   device=device),
+                            status=torch.empty(n_lc, dtype=torch.int32,
+                                               device=device),
+                            period_offsets=period_offsets)
+
+    def to_host(self):
+        out = {k: getattr(self, k).cpu().numpy() for k in FIELDS}
+        out["best_index"] = self.best_index.cpu().numpy()
+        out["status"] = self.status.cpu().numpy()
+        return out
+
+    def curve(self, i, host=None):
+        """Fields of curve i as numpy arrays (from ``host`` if given)."""
+        h = host if host is not None else self.to_host()
+        a, b = self.period_offsets[i], self.period_offsets[i + 1]
+        return {k: h[k][a:b] for k in FIELDS}
+
+
+class Context:
+    """A libatg_bls context bound to one GPU and one stream."""
+
+    def __init__(self, device=None, stream=None, hbm_budget_bytes=0,
+                 tuning=None):
+        self.device = _dev(device)
+        self._lib = _abi.lib()
+        self.stream = stream if stream is not None else \
+            torch.cuda.current_stream(self.device)
+        h = ctypes.c_void_p()
+        rc = self._lib.atg_ctx_create(self.device.index,
+                                      ctypes.c_void_p(self.stream.cuda_stream),
+                                      int(hbm_budget_bytes), ctypes.byref(h))
+        if rc != _abi.ATG_OK:
+            raise _abi.AtgError(rc, "context creation failed on %s (gfx950 "
+                                "required)" % self.device)
+        self._h = h
+        if tuning:
+            self.set_tuning(**tuning)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.atg_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != _abi.ATG_OK:
+            msg = self._lib.atg_last_error(self._h).decode()
+            if rc == _abi.ATG_ERR_BUDGET:
+                raise _abi.MemoryBudgetError(rc, msg)
+            raise _abi.AtgError(rc, msg)
+
+    def set_stream(self, stream):
+        self.stream = stream
+        self._check(self._lib.atg_ctx_set_stream(
+            self._h, ctypes.c_void_p(stream.cuda_stream)))
+
+    def set_tuning(self, periods_per_item=0, lds_bins=0, wg_per_cu=0):
+        t = _abi.Tuning(int(periods_per_item), int(lds_bins), int(wg_per_cu),
+                        0)
+        self._check(self._lib.atg_ctx_set_tuning(self._h, ctypes.byref(t)))
+
+    def get_tuning(self):
+        t = _abi.Tuning()
+        self._check(self._lib.atg_ctx_get_tuning(self._h, ctypes.byref(t)))
+        return dict(periods_per_item=t.periods_per_item, lds_bins=t.lds_bins,
+                    wg_per_cu=t.wg_per_cu)
+
+    def timing(self, enable=True, reset=True):
+        """Average main-kernel duration (ms) over the runs since the last
+        reset, from HIP events recorded on this context's stream."""
+        ms = ctypes.c_double(0.0)
+        n = self._lib.atg_timing_main_kernel(self._h, int(enable), int(reset),
+                                             ctypes.byref(ms))
+        return n, ms.value
+
+    def _structs(self, batch, out, objective, mode, oversample):
+        if objective not in _abi.OBJECTIVES:
+            raise ValueError("objective must be 'likelihood' or 'snr'")
+        if mode not in _abi.MODES:
+            raise ValueError("mode must be 'parity' or 'fast'")
+        for name in ("t", "y", "lc_offsets", "periods", "period_offsets",
+                     "durations", "dur_offsets"):
+            x = getattr(batch, name)
+            if x.device != self.device or not x.is_contiguous():
+                raise ValueError("%s must be contiguous on %s" % (name,
+                                                                  self.device))
+        dt = {torch.float32: _abi.ATG_F32, torch.float64: _abi.ATG_F64}
+        if batch.y.dtype not in dt or (batch.dy is not None and
+                                       batch.dy.dtype not in dt):
+            raise ValueError("y and dy must be float32 or float64")
+        if batch.t.dtype != torch.float64:
+            raise ValueError("t must be float64")
+        b = _abi.Batch(
+            n_lc=batch.n_lc, n_points=int(batch.t.numel()),
+            n_periods=batch.n_periods,
+            n_durations=int(batch.durations.numel()),
+            t=_ptr(batch.t), y=_ptr(batch.y), dy=_ptr(batch.dy),
+            y_dtype=dt[batch.y.dtype],
+            dy_dtype=dt[batch.dy.dtype] if batch.dy is not None else 0,
+            lc_offsets=_ptr(batch.lc_offsets), periods=_ptr(batch.periods),
+            period_offsets=_ptr(batch.period_offsets),
+            durations=_ptr(batch.durations),
+            dur_offsets=_ptr(batch.dur_offsets),
+            oversample=int(oversample),
+            objective=_abi.OBJECTIVES[objective], mode=_abi.MODES[mode],
+            reserved=0)
+        o = _abi.Out(*[_ptr(getattr(out, k)) for k in FIELDS],
+                     best_index=_ptr(out.best_index), status=_ptr(out.status))
+        return b, o
+
+    def run(self, batch: DeviceBatch, objective="likelihood", mode="parity",
+            oversample=10, out: Optional[DeviceResult] = None, sync=True):
+        """Run the periodogram of ``batch``; returns a DeviceResult (device
+        tensors).  With sync=False the work is only enqueued."""
+        if out is None:
+            out = DeviceResult.empty(batch.n_periods, batch.n_lc,
+                                     batch.host_period_offsets, self.device)
+        b, o = self._structs(batch, out, objective, mode, oversample)
+        fn = self._lib.atg_bls_run_sync if sync else self._lib.atg_bls_run
+        self._check(fn(self._h, ctypes.byref(b), ctypes.byref(o)))
+        return out
+
+    def workspace_bytes(self, batch: DeviceBatch):
+        b, _ = self._structs(batch, DeviceResult.empty(
+            1, 1, batch.host_period_offsets, self.device), "likelihood",
+            "parity", 10)
+        n = ctypes.c_uint64(0)
+        self._check(self._lib.atg_bls_workspace_size(self._h, ctypes.byref(b),
+                                                     ctypes.byref(n)))
+        return int(n.value)
+
+
+_default_ctx = {}
+
+
+def default_context(device=None) -> Context:
+    """One lazily created context per device on torch's current stream."""
+    d = _dev(device)
+    c = _default_ctx.get(d.index)
+    if c is None:
+        c = Context(d)
+        _default_ctx[d.index] = c
+    s = torch.cuda.current_stream(d)
+    if s.cuda_stream != c.stream.cuda_stream:
+        c.set_stream(s)
+    return c

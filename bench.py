@@ -1,0 +1,179 @@
+"""Benchmark of the BLS hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (one "step"): the BLS periodogram of 1,000 seeded light curves x
+15,000 points (FP64 t, FP32 y/dy, 27 d baseline, 2-min cadence, two random
+gaps, half with an injected box) over 5,000 log-uniform periods
+0.5-13.5 d x 5 durations 0.04-0.20 d, likelihood objective, parity mode,
+inputs resident in HBM, one batched launch sequence per step.
+
+Printed beside the headline: the plain one-curve-per-call GPU form (the
+API, host arrays in and out, as the reference's per-curve loop), fast mode,
+the continuous-batching server (LC/s with p50/p99 latency), the roofline of
+the main kernel (VALU lane-ops/s and LDS bytes/s from its average duration,
+timed with HIP events on its stream) and the FP64 CPU oracle on the host
+cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("LC/s at 15k pts x 5k periods x 5 durs vs plain GPU; p50/p99 "
+          "latency")
+
+# MI355X public peaks: 256 CUs at 2.4 GHz, 64 lanes per CU per clock for
+# FP64/FP32 non-packed VALU ops (FP64 vector 78.6 TFLOP/s = 39.3 T FMA/s),
+# LDS 128 B/clk/CU.
+PEAK_VALU = 256 * 64 * 2.4e9          # lane-ops/s
+PEAK_LDS = 256 * 128 * 2.4e9          # bytes/s
+PEAK_HBM = 8.0e12                     # bytes/s
+
+
+def algorithmic_work(n_points, periods, durations, oversample=10):
+    """Per-curve algorithmic work of the path (bin rule of the oracle):
+    folds = N*P; box evaluations E = sum_p sum_k (n_bins(p) - m_k + 1);
+    VALU lane-ops = 6*folds + 4*sum n_bins + 16*E; LDS bytes with 8-byte
+    bins = folds*2*2*8 + sum_p n_bins(p)*(2*2*8 + (1+D)*2*8)."""
+    bd = np.min(durations) / oversample
+    nb = np.ceil(periods / bd).astype(np.int64) + oversample
+    m = np.round(np.asarray(durations) / bd).astype(np.int64)
+    E = int(np.sum(np.maximum(nb[:, None] - m[None, :] + 1, 0)))
+    folds = int(n_points) * periods.size
+    D = len(durations)
+    valu = 6 * folds + 4 * int(nb.sum()) + 16 * E
+    lds = folds * 2 * 2 * 8 + int(nb.sum()) * (2 * 2 * 8 + (1 + D) * 2 * 8)
+    return dict(folds=folds, box_evals=E, valu_lane_ops=valu, lds_bytes=lds,
+                sum_bins=int(nb.sum()))
+
+
+def cpu_baseline(d, n_sample, objective):
+    from oracle import oracle
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    s = time.perf_counter()
+    for i in range(n_sample):
+        oracle.power(d["t"][i], d["y"][i], d["dy"][i], d["periods"],
+                     d["durations"], objective, n_threads=cores)
+    el = time.perf_counter() - s
+    return dict(value=n_sample / el, unit="LC/s", cores=cores, kind="port",
+                sample="%d of the 1,000 curves, full 5,000x5 grid, FP64 C "
+                "oracle (OpenMP over periods), %.1f s" % (n_sample, el))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n-lc", type=int, default=1000)
+    ap.add_argument("--plain", type=int, default=20,
+                    help="curves timed in the one-curve-per-call form")
+    ap.add_argument("--cpu-sample", type=int, default=8)
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--skip-extras", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this engine has no multi-GPU "
+              "path (curves and periods shard independently; see the design "
+              "notes).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from astro_transit_gpu_amd import (BoxLeastSquaresGPU, Context,
+                                       DeviceBatch, synth)
+
+    objective = "likelihood"
+    d = synth.config2(n_lc=args.n_lc, seed=2)
+    ctx = Context(0)
+    batch = DeviceBatch.from_host(d["t"], d["y"], d["dy"], d["periods"],
+                                  d["durations"], device=ctx.device)
+    out = ctx.run(batch, objective=objective, mode="parity")
+
+    def timed(mode, steps, warmup):
+        for _ in range(warmup):
+            ctx.run(batch, objective=objective, mode=mode, out=out, sync=False)
+        torch.cuda.synchronize()
+        ctx.timing(True, True)
+        s = time.perf_counter()
+        for _ in range(steps):
+            ctx.run(batch, objective=objective, mode=mode, out=out, sync=False)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - s
+        n, kms = ctx.timing(False, True)
+        return el, kms
+
+    el, kms = timed("parity", args.steps, args.warmup)
+    value = args.n_lc * args.steps / el
+    work = algorithmic_work(15000, d["periods"], d["durations"])
+    per_launch_valu = work["valu_lane_ops"] * args.n_lc
+    per_launch_lds = work["lds_bytes"] * args.n_lc
+    ach_valu = per_launch_valu / (kms * 1e-3)
+    ach_lds = per_launch_lds / (kms * 1e-3)
+    hbm_in = args.n_lc * 15000 * (8 + 4 + 4)
+    hbm_out = args.n_lc * 5000 * 7 * 8
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "LC/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+i64",
+        "data": "synthetic (seeded, numpy default_rng seed 2)",
+        "config": {"workload": "1,000 curves x 15,000 pts (FP64 t, FP32 "
+                   "y/dy) x 5,000 log-uniform periods 0.5-13.5 d x 5 "
+                   "durations 0.04-0.20 d, likelihood, parity mode",
+                   "n_lc": args.n_lc, "n_points": 15000, "n_periods": 5000,
+                   "n_durations": 5, "objective": objective,
+                   "mode": "parity"},
+        "roofline": {
+            "bound": "valu", "achieved": round(ach_valu / 1e12, 3),
+            "peak": round(PEAK_VALU / 1e12, 3), "unit": "Tlane-op/s",
+            "frac": round(ach_valu / PEAK_VALU, 4), "traffic": None,
+            "kernel": "k_main", "kernel_ms": round(kms, 3),
+            "valu_lane_ops_per_launch": per_launch_valu,
+            "lds": {"achieved_GBps": round(ach_lds / 1e9, 1),
+                    "peak_GBps": PEAK_LDS / 1e9,
+                    "frac": round(ach_lds / PEAK_LDS, 4),
+                    "bytes_per_launch": per_launch_lds},
+            "hbm_algorithmic_bytes_per_launch": hbm_in + hbm_out,
+        },
+    }
+    if not args.skip_extras:
+        el_f, kms_f = timed("fast", max(2, args.steps // 2), 1)
+        line["fast"] = {"value": round(args.n_lc * max(2, args.steps // 2) /
+                                       el_f, 2), "unit": "LC/s",
+                        "kernel_ms": round(kms_f, 3)}
+        # plain form: one curve per call, host arrays in and out
+        lat = []
+        for i in range(args.plain + 1):
+            s = time.perf_counter()
+            BoxLeastSquaresGPU(d["t"][i], d["y"][i], d["dy"][i]).power(
+                d["periods"], d["durations"], method="parity",
+                objective=objective)
+            if i:
+                lat.append(time.perf_counter() - s)
+        plain = len(lat) / sum(lat)
+        line["plain_gpu"] = {"value": round(plain, 2), "unit": "LC/s",
+                             "p50_ms": round(np.percentile(lat, 50) * 1e3, 3),
+                             "p99_ms": round(np.percentile(lat, 99) * 1e3, 3),
+                             "curves": len(lat)}
+        line["batched_over_plain"] = round(value / plain, 2)
+        try:
+            from astro_transit_gpu_amd.serving import serve_benchmark
+            line["served"] = serve_benchmark(d, seconds=args.serve_seconds,
+                                             batched_rate=value,
+                                             objective=objective)
+        except ImportError:
+            line["served"] = None
+        line["cpu_baseline"] = cpu_baseline(d, args.cpu_sample, objective)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

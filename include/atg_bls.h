@@ -1,0 +1,171 @@
+/*
+ * atg_bls.h -- C ABI of the MI355X Box Least Squares engine
+ * (libatg_bls.so, built for gfx950).
+ *
+ * One call runs the BLS periodogram over a ragged batch of light curves
+ * that already sit in device memory:
+ *   per light curve:  t (FP64), y and dy (FP32 or FP64), its own period
+ *                     grid and duration set (CSR offsets);
+ *   per period:       power, depth, depth_err, duration, transit_time,
+ *                     depth_snr, log_likelihood (FP64);
+ *   per light curve:  index of the best period and a status code.
+ *
+ * The results follow astropy's BoxLeastSquares.power(method="fast")
+ * (the reference's oracle, src/astrotransit_gpu/search/cpu_reference_bls.py:
+ * 11-16): t_ref = min(t), ivar = 1/dy^2, y - median(y), bins of
+ * min(durations)/oversample, the likelihood or snr objective.
+ *
+ * Conventions: every call returns an int status (ATG_OK = 0, negative on
+ * failure; atg_last_error() gives the message).  Nothing throws across the
+ * ABI.  All device pointers belong to the caller; the library never frees
+ * them.  Work is enqueued on the context's stream and is asynchronous
+ * unless the function name ends in _sync.  A context is used by one host
+ * thread at a time.
+ */
+#ifndef ATG_BLS_H
+#define ATG_BLS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATG_ABI_VERSION 1
+
+/* return codes */
+#define ATG_OK 0
+#define ATG_ERR_ARG (-1)      /* bad argument (null pointer, size, enum)   */
+#define ATG_ERR_BUDGET (-2)   /* workspace would exceed the HBM budget     */
+#define ATG_ERR_HIP (-3)      /* HIP runtime or launch failure             */
+#define ATG_ERR_LDS (-4)      /* a period needs more bins than LDS holds   */
+#define ATG_ERR_DEVICE (-5)   /* device is not gfx950 / not available      */
+
+/* per-light-curve status written by the device */
+#define ATG_LC_OK 0
+#define ATG_LC_BAD_PERIOD 1   /* a period < DBL_EPSILON (astropy code 1)   */
+#define ATG_LC_BAD_DURATION 2 /* max duration > min period, or a duration
+                                 < DBL_EPSILON (astropy code 2)            */
+#define ATG_LC_NAN 3          /* NaN in t or y                             */
+#define ATG_LC_BAD_DY 4       /* dy <= 0 or NaN                            */
+#define ATG_LC_TOO_MANY_BINS 5 /* a period's bins exceed the LDS budget    */
+#define ATG_LC_EMPTY 6        /* no points, periods or durations           */
+#define ATG_LC_TOO_MANY_DURATIONS 7 /* more than 64 durations             */
+
+/* element types */
+#define ATG_F32 1
+#define ATG_F64 2
+
+/* objective: astropy power(objective=...) */
+#define ATG_OBJ_LIKELIHOOD 0
+#define ATG_OBJ_SNR 1
+
+/* mode: reference power(method=...) */
+#define ATG_MODE_PARITY 0 /* int64 fixed-point bins: bit-reproducible      */
+#define ATG_MODE_FAST 1   /* FP32 bins with LDS float atomics              */
+
+typedef struct atg_ctx atg_ctx;
+
+/* A ragged batch of n_lc light curves; every pointer is a device pointer.
+ * Light curve i owns points [lc_offsets[i], lc_offsets[i+1]), periods
+ * [period_offsets[i], period_offsets[i+1]) and durations
+ * [dur_offsets[i], dur_offsets[i+1]). */
+typedef struct atg_bls_batch {
+    int64_t n_lc;
+    int64_t n_points;        /* total points   = lc_offsets[n_lc]          */
+    int64_t n_periods;       /* total periods  = period_offsets[n_lc]      */
+    int64_t n_durations;     /* total durations = dur_offsets[n_lc]        */
+    const double *t;         /* [n_points] FP64 times                      */
+    const void *y;           /* [n_points] flux, y_dtype                   */
+    const void *dy;          /* [n_points] flux error, dy_dtype; NULL = 1  */
+    int32_t y_dtype;         /* ATG_F32 / ATG_F64                          */
+    int32_t dy_dtype;        /* ATG_F32 / ATG_F64 (ignored if dy == NULL)  */
+    const int64_t *lc_offsets;     /* [n_lc + 1]                           */
+    const double *periods;         /* [n_periods]                          */
+    const int64_t *period_offsets; /* [n_lc + 1]                           */
+    const double *durations;       /* [n_durations]                        */
+    const int64_t *dur_offsets;    /* [n_lc + 1]                           */
+    int32_t oversample;      /* bins per minimum duration (astropy: 10)    */
+    int32_t objective;       /* ATG_OBJ_*                                  */
+    int32_t mode;            /* ATG_MODE_*                                 */
+    int32_t reserved;
+} atg_bls_batch;
+
+/* Caller-allocated device outputs aligned with the periods of the batch. */
+typedef struct atg_bls_out {
+    double *power;           /* [n_periods]                                */
+    double *depth;
+    double *depth_err;
+    double *duration;
+    double *transit_time;    /* absolute: phase + t_ref                    */
+    double *depth_snr;
+    double *log_likelihood;
+    int64_t *best_index;     /* [n_lc] argmax of power within the curve's
+                                grid (first maximum); may be NULL          */
+    int32_t *status;         /* [n_lc] ATG_LC_*; may be NULL               */
+} atg_bls_out;
+
+/* Kernel tiling (chosen by the autotuner; 0 fields keep the current value).
+ */
+typedef struct atg_tuning {
+    int32_t periods_per_item; /* periods per work item (1..8)              */
+    int32_t lds_bins;         /* bins per LDS pass (sets dynamic LDS size) */
+    int32_t wg_per_cu;        /* persistent workgroups per CU              */
+    int32_t reserved;
+} atg_tuning;
+
+/* ABI version (ATG_ABI_VERSION). */
+int atg_abi_version(void);
+
+/* Creates a context on `device` that enqueues on `stream` (a hipStream_t;
+ * NULL = the null stream) and caps the library's own device workspace at
+ * `hbm_budget_bytes` (0 = no cap).  Replaces the per-call CuPy set-up of the
+ * reference wrappers (search/gpu_bls.py:18-40, search/v42_parity.py:7-25:
+ * NVRTC compile and module cache): kernels here are compiled ahead of time.
+ */
+int atg_ctx_create(int device, void *stream, uint64_t hbm_budget_bytes,
+                   atg_ctx **out);
+void atg_ctx_destroy(atg_ctx *ctx);
+
+/* Replaces the stream the context enqueues on. */
+int atg_ctx_set_stream(atg_ctx *ctx, void *stream);
+
+/* Last error message of the context (never NULL). */
+const char *atg_last_error(const atg_ctx *ctx);
+
+/* Tiling of the main kernel. */
+int atg_ctx_set_tuning(atg_ctx *ctx, const atg_tuning *tuning);
+int atg_ctx_get_tuning(const atg_ctx *ctx, atg_tuning *tuning);
+
+/* Device workspace a batch of this shape needs, in bytes. */
+int atg_bls_workspace_size(const atg_ctx *ctx, const atg_bls_batch *batch,
+                           uint64_t *bytes);
+
+/* Runs the periodogram of the batch (asynchronous on the context stream).
+ * Replaces the reference's kernel launches behind
+ * BoxLeastSquaresGPU.power: run_vbls_exact_parity's
+ * vbls_v42_parity_kernel launch (search/v42_parity.py:27-107, kernel
+ * search/kernels/vbls_v42_parity.cu:13-31) for mode PARITY and
+ * run_gpu_bls's bls_kernel launch (search/gpu_bls.py:42-144, kernel
+ * search/kernels/bls.cu:12-27) for mode FAST, including their host-side
+ * preparation (weights, totals, t_start) and argmax. */
+int atg_bls_run(atg_ctx *ctx, const atg_bls_batch *batch,
+                const atg_bls_out *out);
+
+/* atg_bls_run followed by a stream synchronisation; also reports the
+ * first non-OK light-curve status as ATG_ERR_LDS when a period needed more
+ * bins than the LDS budget (other statuses are left to the caller). */
+int atg_bls_run_sync(atg_ctx *ctx, const atg_bls_batch *batch,
+                     const atg_bls_out *out);
+
+/* Average duration (ms) of the main kernel over the runs since the last
+ * reset, timed with HIP events on the context stream; resets when
+ * `reset` != 0.  Returns the number of timed runs (0 = none). */
+int atg_timing_main_kernel(atg_ctx *ctx, int enable, int reset,
+                           double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ATG_BLS_H */
