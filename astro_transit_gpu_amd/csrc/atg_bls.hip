@@ -93,8 +93,10 @@ struct Work {
     double *ws_t;         // lane-major t'
     long long *ws_w;      // lane-major quantized ivar
     long long *ws_y;      // lane-major quantized ivar * (y - median)
-    int *ncand;           // [n_lc] periods refined in the oracle's order
-    long long *cand;      // [n_lc * CAP] their global period indices
+    unsigned int *cand_count;  // periods queued for refinement
+    unsigned int *cand_next;   // refinement queue head
+    int *cand_lc;         // [n_lc * CAP] curve of each queued period
+    long long *cand_p;    // [n_lc * CAP] its global period index
 };
 
 // ---------------------------------------------------------------- helpers
@@ -103,6 +105,15 @@ __device__ __forceinline__ double load_val(const void *p, int dt,
                                            long long i) {
     return dt == ATG_F32 ? (double)((const float *)p)[i]
                          : ((const double *)p)[i];
+}
+
+// Value of lane j (uniform j) of an FP64 register, as a wave-uniform value.
+__device__ __forceinline__ double readlane_d(double v, int j) {
+    const long long x = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)x, j);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(x >> 32), j);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) |
+                                            lo));
 }
 
 // Order-preserving unsigned key of an FP64 value (non-NaN).
@@ -402,6 +413,8 @@ __global__ __launch_bounds__(NT) void k_items(long long n_lc,
     if (threadIdx.x == 0) {
         w.item_off[n_lc] = carry;
         *w.counter = 0u;
+        *w.cand_count = 0u;
+        *w.cand_next = 0u;
     }
 }
 
@@ -909,23 +922,20 @@ constexpr double TIE_REL = 1e-9;   // >> relative error of the int64 path
 __global__ __launch_bounds__(NT) void k_cands(Batch b, Out o, Work w) {
     __shared__ double dscr[NW];
     __shared__ int iscr[NW];
+    __shared__ long long s_c[CAP];
+    __shared__ unsigned int s_base;
     const int lc = blockIdx.x;
     const long long po = b.per_off[lc], np = b.per_off[lc + 1] - po;
     if (b.mode != ATG_MODE_PARITY || w.prm[lc].status != ATG_LC_OK ||
-        w.lc_flags[lc]) {
-        if (threadIdx.x == 0) w.ncand[lc] = 0;
+        w.lc_flags[lc])
         return;
-    }
     double m = -INFINITY;
     for (long long i = threadIdx.x; i < np; i += NT) {
         const double v = o.power[po + i];
         if (v > m) m = v;  // NaN never compares greater
     }
     m = block_reduce(m, MaxOp(), dscr);
-    if (!(m > -INFINITY)) {
-        if (threadIdx.x == 0) w.ncand[lc] = 0;
-        return;
-    }
+    if (!(m > -INFINITY)) return;
     const double thr = m - TIE_REL * fabs(m);
     int count = 0;
     for (long long base = 0; base < np && count < CAP; base += NT) {
@@ -933,10 +943,17 @@ __global__ __launch_bounds__(NT) void k_cands(Batch b, Out o, Work w) {
         const int f = (i < np && o.power[po + i] >= thr) ? 1 : 0;
         int tot;
         const int e = block_excl_scan(f, iscr, &tot);
-        if (f && count + e < CAP) w.cand[(long long)lc * CAP + count + e] = po + i;
+        if (f && count + e < CAP) s_c[count + e] = po + i;
         count += tot;
     }
-    if (threadIdx.x == 0) w.ncand[lc] = min(count, CAP);
+    count = min(count, CAP);
+    __syncthreads();
+    if (threadIdx.x == 0) s_base = atomicAdd(w.cand_count, (unsigned)count);
+    __syncthreads();
+    if (threadIdx.x < count) {
+        w.cand_lc[s_base + threadIdx.x] = lc;
+        w.cand_p[s_base + threadIdx.x] = s_c[threadIdx.x];
+    }
 }
 
 __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
@@ -944,14 +961,19 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
     extern __shared__ __align__(16) unsigned char smem[];
     double *my = (double *)smem;        // sum of y*ivar per bin
     double *mi = my + lds_bins;         // sum of ivar per bin
-    __shared__ int s_ind[64];
-    __shared__ double s_yv[64], s_iv[64];
     __shared__ int s_m[DMAX];
     __shared__ double s_tot[2];
-    const int lc = blockIdx.y, c = blockIdx.x, lane = threadIdx.x;
-    if (c >= w.ncand[lc]) return;
+    __shared__ unsigned int s_q;
+    const int lane = threadIdx.x;
+    for (;;) {
+    __syncthreads();
+    if (lane == 0) s_q = atomicAdd(w.cand_next, 1u);
+    __syncthreads();
+    const unsigned int q = s_q;
+    if (q >= *w.cand_count) break;
+    const int lc = w.cand_lc[q];
     const LcParams P = w.prm[lc];
-    const long long p = w.cand[(long long)lc * CAP + c];
+    const long long p = w.cand_p[q];
     const double period = b.periods[p];
     const double bd = P.bd, inv_bd = 1.0 / bd, invP = 1.0 / period;
     const int os = b.oversample;
@@ -966,7 +988,7 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
     // segments of start positions with the padded destinations of each
     // point and the exact FP64 prefix carried from segment to segment
     const bool fits = nb + 1 <= lds_bins;
-    if (!fits && !(nb - os > os && lds_bins - mmax - 1 >= 1)) return;
+    if (!fits && !(nb - os > os && lds_bins - mmax - 1 >= 1)) continue;
     const int S = fits ? nb + 1 : lds_bins - mmax - 1;
     const bool like = b.objective == ATG_OBJ_LIKELIHOOD;
     const int nval = nb + 1;
@@ -983,61 +1005,73 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
         __syncthreads();
         double ay = 0.0, ai = 0.0;
         int cur = -1;
-        for (int base = 0; base < n; base += 64) {
-            const int i = base + lane;
-            if (i < n) {
-                const double tp = b.t[off + i] - P.t_ref;
-                double iv = 1.0;
-                if (b.dy) {
-                    const double d = load_val(b.dy, b.dy_dt, off + i);
-                    iv = 1.0 / (d * d);
-                }
-                const double yc = load_val(b.y, b.y_dt, off + i) - P.y_med;
-                s_yv[lane] = yc * iv;
-                s_iv[lane] = iv;
-                s_ind[lane] = min(max(fold_bin(tp, period, invP, bd, inv_bd),
-                                      0), nb);
-            }
-            __syncthreads();
-            if (lane == 0) {
-                const int cnt = min(64, n - base);
-                for (int j = 0; j < cnt; ++j) {
-                    const double yv = s_yv[j], iv = s_iv[j];
-                    const int ind = s_ind[j];
-                    if (s0 == 0) {
-                        sum_y += yv;
-                        sum_iv += iv;
-                    }
-                    if (fits) {
-                        // the bin's running sum stays in registers while
-                        // consecutive points share it: same order of adds
-                        if (ind != cur) {
-                            if (cur >= 0) {
-                                my[cur] = ay;
-                                mi[cur] = ai;
-                            }
-                            cur = ind;
-                            ay = my[ind];
-                            ai = mi[ind];
-                        }
-                        ay += yv;
-                        ai += iv;
-                    } else {
-                        int j1, j2;
-                        pad_targets(ind, nb, os, j1, j2);
-                        if (j1 >= s0 && j1 <= hi) {
-                            my[j1 - s0] += yv;
-                            mi[j1 - s0] += iv;
-                        }
-                        if (j2 >= s0 && j2 <= hi) {
-                            my[j2 - s0] += yv;
-                            mi[j2 - s0] += iv;
-                        }
-                    }
-                }
-            }
-            __syncthreads();
+        // raw inputs of the first chunk; each later chunk is loaded while
+        // the wave walks the current one
+        double t_raw = 0.0, y_raw = 0.0, d_raw = 1.0;
+        if (lane < n) {
+            t_raw = b.t[off + lane];
+            y_raw = load_val(b.y, b.y_dt, off + lane);
+            if (b.dy) d_raw = load_val(b.dy, b.dy_dt, off + lane);
         }
+        for (int base = 0; base < n; base += 64) {
+            const double tp = t_raw - P.t_ref;
+            const double iv = b.dy ? 1.0 / (d_raw * d_raw) : 1.0;
+            const double yv = (y_raw - P.y_med) * iv;
+            const int ind = min(max(fold_bin(tp, period, invP, bd, inv_bd), 0),
+                                nb);
+            const int nx = base + 64 + lane;
+            if (nx < n) {
+                t_raw = b.t[off + nx];
+                y_raw = load_val(b.y, b.y_dt, off + nx);
+                if (b.dy) d_raw = load_val(b.dy, b.dy_dt, off + nx);
+            }
+            // the wave walks the chunk in point order; every lane holds the
+            // same running values, lane 0 stores them
+            const int cnt = min(64, n - base);
+            for (int j = 0; j < cnt; ++j) {
+                const double yvj = readlane_d(yv, j), ivj = readlane_d(iv, j);
+                const int indj = __builtin_amdgcn_readlane(ind, j);
+                if (s0 == 0) {
+                    sum_y += yvj;
+                    sum_iv += ivj;
+                }
+                if (fits) {
+                    // the bin's running sum stays in registers while
+                    // consecutive points share it: same order of adds
+                    if (indj != cur) {
+                        if (cur >= 0 && lane == 0) {
+                            my[cur] = ay;
+                            mi[cur] = ai;
+                        }
+                        cur = indj;
+                        ay = my[indj];
+                        ai = mi[indj];
+                    }
+                    ay += yvj;
+                    ai += ivj;
+                } else {
+                    int j1, j2;
+                    pad_targets(indj, nb, os, j1, j2);
+                    if (j1 >= s0 && j1 <= hi) {
+                        const double a = my[j1 - s0] + yvj;
+                        const double bb = mi[j1 - s0] + ivj;
+                        if (lane == 0) {
+                            my[j1 - s0] = a;
+                            mi[j1 - s0] = bb;
+                        }
+                    }
+                    if (j2 >= s0 && j2 <= hi) {
+                        const double a = my[j2 - s0] + yvj;
+                        const double bb = mi[j2 - s0] + ivj;
+                        if (lane == 0) {
+                            my[j2 - s0] = a;
+                            mi[j2 - s0] = bb;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
         if (lane == 0) {
             if (fits) {
                 if (cur >= 0) {
@@ -1132,6 +1166,7 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
         o.depth_snr[p] = snr;
         o.log_like[p] = ll;
     }
+    }  // queue loop
 }
 
 // ----------------------------------------------------------------- k_final
@@ -1235,7 +1270,7 @@ int hip_fail(atg_ctx *c, hipError_t e, const char *what) {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct Layout {
-    size_t prm, flags, items, counter, ncand, cand, t, w, y, total;
+    size_t prm, flags, items, counter, cand_lc, cand_p, t, w, y, total;
 };
 
 Layout layout_of(const atg_bls_batch *b) {
@@ -1246,9 +1281,9 @@ Layout layout_of(const atg_bls_batch *b) {
     L.prm = o;     o = align256(o + nlc * sizeof(LcParams));
     L.flags = o;   o = align256(o + nlc * sizeof(int));
     L.items = o;   o = align256(o + (nlc + 1) * sizeof(long long));
-    L.counter = o; o = align256(o + sizeof(unsigned int));
-    L.ncand = o;   o = align256(o + nlc * sizeof(int));
-    L.cand = o;    o = align256(o + nlc * CAP * sizeof(long long));
+    L.counter = o; o = align256(o + 3 * sizeof(unsigned int));
+    L.cand_lc = o; o = align256(o + nlc * CAP * sizeof(int));
+    L.cand_p = o;  o = align256(o + nlc * CAP * sizeof(long long));
     L.t = o;       o = align256(o + slots * sizeof(double));
     L.w = o;       o = align256(o + slots * sizeof(long long));
     L.y = o;       o = align256(o + slots * sizeof(long long));
@@ -1433,8 +1468,10 @@ int atg_bls_run(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o) {
     w.ws_t = (double *)(base + L.t);
     w.ws_w = (long long *)(base + L.w);
     w.ws_y = (long long *)(base + L.y);
-    w.ncand = (int *)(base + L.ncand);
-    w.cand = (long long *)(base + L.cand);
+    w.cand_count = w.counter + 1;
+    w.cand_next = w.counter + 2;
+    w.cand_lc = (int *)(base + L.cand_lc);
+    w.cand_p = (long long *)(base + L.cand_p);
     Batch kb = to_batch(b);
     Out ko;
     ko.power = o->power;
@@ -1486,8 +1523,8 @@ int atg_bls_run(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o) {
         c->pending.emplace_back(ev0, ev1);
     }
     hipLaunchKernelGGL(k_cands, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w);
-    hipLaunchKernelGGL(k_exact, dim3(CAP, nlc), dim3(64), lds, c->stream, kb,
-                       ko, w, c->tune.lds_bins);
+    hipLaunchKernelGGL(k_exact, dim3(c->num_cu * 2), dim3(64), lds, c->stream,
+                       kb, ko, w, c->tune.lds_bins);
     hipLaunchKernelGGL(k_final, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(c, e, "kernel launch");
