@@ -199,6 +199,75 @@ __device__ __forceinline__ int fold_bin(double t, double P, double invP,
     return (int)xi + 1;
 }
 
+// Branch-free form of fold_bin for the hot loop: the bin from the rounded
+// reciprocals, plus a flag raised when either quotient lies within 2^-50
+// (relative) of an integer, the only case where the result can differ from
+// the oracle's divisions.  Flagged points are re-binned with fold_exact.
+__device__ __forceinline__ int fold_fast(double t, double P, double invP,
+                                         double inv_bd, bool &flag) {
+    const double r = t * invP;
+    const double q = floor(r);
+    const double fr = r - q;
+    const double tol = r * 0x1p-50;
+    const double ph = t - P * q;
+    const double x = ph * inv_bd;
+    const double xi = trunc(x);
+    const double fx = x - xi;
+    const double tolx = fabs(x) * 0x1p-50;
+    flag = (fr <= tol) | (fr >= 1.0 - tol) | (fx <= tolx) | (fx >= 1.0 - tolx);
+    return (int)xi + 1;
+}
+
+// The oracle's bin rule with true divisions.
+__device__ __forceinline__ int fold_exact(double t, double P, double bd) {
+    const double ph = t - P * floor(t / P);
+    return (int)trunc(ph / bd) + 1;
+}
+
+// Bins every point of the lane for C periods at once.  C is a compile-time
+// constant so that the C fold chains are independent straight-line code the
+// scheduler can interleave; the next point's loads are issued before the
+// current point is folded.  Bins that a near-integer quotient makes
+// uncertain are recomputed with true divisions before the atomics.
+template <int C>
+__device__ __forceinline__ void bin_points(
+    const double *tp, const long long *wp, const long long *yp, int L,
+    const double *per, const double *inv, const int *nb, const int *base,
+    double bd, double inv_bd, long long *binW, long long *binY) {
+    double tv_n = tp[0];
+    long long wv_n = wp[0], yv_n = yp[0];
+    for (int i = 0; i < L; ++i) {
+        const double tv = tv_n;
+        const long long wv = wv_n, yv = yv_n;
+        if (i + 1 < L) {
+            tv_n = tp[(long long)(i + 1) * NT];
+            wv_n = wp[(long long)(i + 1) * NT];
+            yv_n = yp[(long long)(i + 1) * NT];
+        }
+        int ind[C];
+        unsigned flags = 0;
+#pragma unroll
+        for (int s = 0; s < C; ++s) {
+            bool f;
+            ind[s] = fold_fast(tv, per[s], inv[s], inv_bd, f);
+            flags |= (unsigned)f << s;
+        }
+        if (flags) {
+#pragma unroll
+            for (int s = 0; s < C; ++s)
+                if ((flags >> s) & 1u) ind[s] = fold_exact(tv, per[s], bd);
+        }
+#pragma unroll
+        for (int s = 0; s < C; ++s) {
+            const int at = base[s] + min(max(ind[s], 0), nb[s]);
+            __hip_atomic_fetch_add(&binW[at], wv, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(&binY[at], yv, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ k_prep
 
 // Two order statistics k0 <= k1 of the curve's y values (as FP64) by an
@@ -429,6 +498,32 @@ __device__ __forceinline__ bool better(double s, int i, double bs, int bi) {
     return s > bs || (s == bs && i < bi);
 }
 
+// Running best window of the box scan.  A window's score is q/den with
+// q = num^2 (num = depth * W_in * W_out / W_tot^2) and den = w_in * w_out^2
+// (likelihood) or w_in * w_out (snr), a monotone transform of the oracle's
+// objective; scores are compared by cross-multiplication (no division),
+// equal scores keep the lowest (duration, start) index as the oracle's
+// first-wins loop does.
+struct Score {
+    double q = -1.0, den = 1.0;
+    int idx = INT_MAX;
+    __device__ __forceinline__ void offer(double q2, double d2, int i2) {
+        const double lhs = q2 * den, rhs = q * d2;
+        if (lhs > rhs || (lhs == rhs && i2 < idx)) {
+            q = q2;
+            den = d2;
+            idx = i2;
+        }
+    }
+    __device__ __forceinline__ void wave_reduce() {
+        for (int o = 32; o > 0; o >>= 1) {
+            const double oq = __shfl_xor(q, o), od = __shfl_xor(den, o);
+            const int oi = __shfl_xor(idx, o);
+            offer(oq, od, oi);
+        }
+    }
+};
+
 __device__ __forceinline__ void write_nan(const Out &o, long long p) {
     const double nan = __longlong_as_double(0x7ff8000000000000ll);
     o.power[p] = nan;
@@ -475,21 +570,23 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
     long long totw, toty;
     long long ow = block_excl_scan(sw, lscr, &totw);
     long long oy = block_excl_scan(sy, lscr, &toty);
+    // cumulative sums in units of the total weight (FP64)
     double *Cw = (double *)binW;
     double *Cy = (double *)binY;
+    const double fw = P.w_unit / P.W_tot, fy = P.y_unit / P.W_tot;
     for (int i = b0; i < b1; ++i) {
         ow += binW[base + i];
         oy += binY[base + i];
-        Cw[base + i] = (double)ow * P.w_unit;
-        Cy[base + i] = (double)oy * P.y_unit;
+        Cw[base + i] = (double)ow * fw;
+        Cy[base + i] = (double)oy * fy;
     }
     __syncthreads();
 
     // box scan: start n, duration k (m bins), window (n, n+m]
     const double Wt = P.W_tot, Yt = P.Y_tot;
+    const double ytn = Yt / Wt, epsn = DBL_EPSILON / Wt;
     const bool like = b.objective == ATG_OBJ_LIKELIHOOD;
-    double bs = -1.0;
-    int bi = INT_MAX;
+    Score best;
     for (int s = tid; s < nb; s += NT) {
         const double w0 = Cw[base + s], y0 = Cy[base + s];
         for (int k = 0; k < nd; ++k) {
@@ -497,42 +594,28 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
             if (s + m > nb) continue;
             const double win = Cw[base + s + m] - w0;
             const double yin = Cy[base + s + m] - y0;
-            const double wout = Wt - win;
-            if (win < DBL_EPSILON || wout < DBL_EPSILON) continue;
-            // depth * W_in * W_out = Y_tot*W_in - Y_in*W_tot
-            const double num = fma(Yt, win, -(yin * Wt));
+            const double wout = 1.0 - win;
+            if (win < epsn || wout < epsn) continue;
+            // depth * W_in * W_out / W_tot^2 = y_tot*w_in - y_in
+            const double num = fma(ytn, win, -yin);
             if (num < 0.0) continue;
-            const double den = like ? win * wout * wout : win * wout;
-            const double sc = (num * num) / den;
-            const int idx = k * nval + s;
-            if (better(sc, idx, bs, bi)) {
-                bs = sc;
-                bi = idx;
-            }
+            best.offer(num * num, like ? win * wout * wout : win * wout,
+                       k * nval + s);
         }
     }
     // workgroup argmax (highest score, then lowest (k, n))
-    for (int o = 32; o > 0; o >>= 1) {
-        double os_ = __shfl_xor(bs, o);
-        int oi = __shfl_xor(bi, o);
-        if (better(os_, oi, bs, bi)) {
-            bs = os_;
-            bi = oi;
-        }
-    }
+    best.wave_reduce();
     const int wv = tid >> 6, l = tid & 63;
     __syncthreads();
     if (l == 0) {
-        dscr[wv] = bs;
-        iscr[wv] = bi;
+        dscr[wv] = best.q;
+        dscr[NW + wv] = best.den;
+        iscr[wv] = best.idx;
     }
     __syncthreads();
     if (tid == 0) {
-        for (int i = 1; i < NW; ++i)
-            if (better(dscr[i], iscr[i], bs, bi)) {
-                bs = dscr[i];
-                bi = iscr[i];
-            }
+        for (int i = 1; i < NW; ++i) best.offer(dscr[i], dscr[NW + i], iscr[i]);
+        const int bi = best.idx;
         if (bi == INT_MAX) {  // no window qualifies: power -inf as astropy
             o.power[p] = -INFINITY;
             o.depth[p] = 0.0;
@@ -543,8 +626,8 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
             o.log_like[p] = 0.0;
         } else {
             const int k = bi / nval, s = bi - k * nval, m = s_m[k];
-            const double win = Cw[base + s + m] - Cw[base + s];
-            const double yin_s = Cy[base + s + m] - Cy[base + s];
+            const double win = (Cw[base + s + m] - Cw[base + s]) * Wt;
+            const double yin_s = (Cy[base + s + m] - Cy[base + s]) * Wt;
             const double wout = Wt - win;
             const double y_in = yin_s / win;
             const double y_out = (Yt - yin_s) / wout;
@@ -609,8 +692,10 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
     const int nval = nb + 1;
     const bool like = b.objective == ATG_OBJ_LIKELIHOOD;
     const double Wt = P.W_tot, Yt = P.Y_tot;
-    double bs = -1.0, bwin = 0.0, byin = 0.0;
-    int bi = INT_MAX;
+    const double ytn = Yt / Wt, epsn = DBL_EPSILON / Wt;
+    const double fw = P.w_unit / Wt, fy = P.y_unit / Wt;
+    Score best;
+    double bwin = 0.0, byin = 0.0;
     const double *tp = w.ws_t + P.ws_off + tid;
     const long long *wp = w.ws_w + P.ws_off + tid;
     const long long *yp = w.ws_y + P.ws_off + tid;
@@ -627,7 +712,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
         for (int i = 0; i < P.L; ++i) {
             const long long wv = wp[(long long)i * NT];
             const long long yv = yp[(long long)i * NT];
-            if (wv == 0 && yv == 0) continue;
+            if ((wv | yv) == 0) continue;
             const double tv = tp[(long long)i * NT];
             const int r = min(max(fold_bin(tv, period, invP, bd, inv_bd), 0),
                               nb);
@@ -652,8 +737,8 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
         for (int i = b0; i < b1; ++i) {
             ow += binW[i];
             oy += binY[i];
-            Cw[i] = (double)ow * P.w_unit;
-            Cy[i] = (double)oy * P.y_unit;
+            Cw[i] = (double)ow * fw;
+            Cy[i] = (double)oy * fy;
         }
         __syncthreads();
         const int nend = min(s0 + S, nb);
@@ -665,16 +750,14 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
                 if (n + m > nb) continue;
                 const double win = Cw[l + m] - w0;
                 const double yin = Cy[l + m] - y0;
-                const double wout = Wt - win;
-                if (win < DBL_EPSILON || wout < DBL_EPSILON) continue;
-                const double num = fma(Yt, win, -(yin * Wt));
+                const double wout = 1.0 - win;
+                if (win < epsn || wout < epsn) continue;
+                const double num = fma(ytn, win, -yin);
                 if (num < 0.0) continue;
-                const double den = like ? win * wout * wout : win * wout;
-                const double sc = (num * num) / den;
-                const int idx = k * nval + n;
-                if (better(sc, idx, bs, bi)) {
-                    bs = sc;
-                    bi = idx;
+                const int before = best.idx;
+                best.offer(num * num, like ? win * wout * wout : win * wout,
+                           k * nval + n);
+                if (best.idx != before) {
                     bwin = win;
                     byin = yin;
                 }
@@ -683,12 +766,12 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
         __syncthreads();
     }
     for (int ofs = 32; ofs > 0; ofs >>= 1) {
-        const double os_ = __shfl_xor(bs, ofs);
-        const int oi = __shfl_xor(bi, ofs);
+        const double oq = __shfl_xor(best.q, ofs), od = __shfl_xor(best.den, ofs);
+        const int oi = __shfl_xor(best.idx, ofs);
         const double ow_ = __shfl_xor(bwin, ofs), oy_ = __shfl_xor(byin, ofs);
-        if (better(os_, oi, bs, bi)) {
-            bs = os_;
-            bi = oi;
+        const int before = best.idx;
+        best.offer(oq, od, oi);
+        if (best.idx != before) {
             bwin = ow_;
             byin = oy_;
         }
@@ -696,20 +779,23 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
     const int wv = tid >> 6, l = tid & 63;
     __syncthreads();
     if (l == 0) {
-        dscr[wv] = bs;
-        iscr[wv] = bi;
+        dscr[wv] = best.q;
+        dscr[NW + wv] = best.den;
+        iscr[wv] = best.idx;
         dscr2[wv] = bwin;
         dscr3[wv] = byin;
     }
     __syncthreads();
     if (tid == 0) {
-        for (int i = 1; i < NW; ++i)
-            if (better(dscr[i], iscr[i], bs, bi)) {
-                bs = dscr[i];
-                bi = iscr[i];
+        for (int i = 1; i < NW; ++i) {
+            const int before = best.idx;
+            best.offer(dscr[i], dscr[NW + i], iscr[i]);
+            if (best.idx != before) {
                 bwin = dscr2[i];
                 byin = dscr3[i];
             }
+        }
+        const int bi = best.idx;
         if (bi == INT_MAX) {
             o.power[p] = -INFINITY;
             o.depth[p] = 0.0;
@@ -720,13 +806,15 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
             o.log_like[p] = 0.0;
         } else {
             const int k = bi / nval, n = bi - k * nval, m = s_m[k];
-            const double wout = Wt - bwin;
-            const double y_in = byin / bwin;
-            const double y_out = (Yt - byin) / wout;
+            const double win = bwin * Wt;
+            const double yin_s = byin * Wt;
+            const double wout = Wt - win;
+            const double y_in = yin_s / win;
+            const double y_out = (Yt - yin_s) / wout;
             const double depth = y_out - y_in;
-            const double derr = sqrt(1.0 / bwin + 1.0 / wout);
+            const double derr = sqrt(1.0 / win + 1.0 / wout);
             const double snr = depth / derr;
-            const double ll = 0.5 * bwin * depth * depth;
+            const double ll = 0.5 * win * depth * depth;
             const double dur = m * P.bd;
             o.power[p] = like ? ll : snr;
             o.depth[p] = depth;
@@ -760,7 +848,7 @@ __global__ __launch_bounds__(NT) void k_main(Batch b, Out o, Work w,
     __shared__ int s_m[DMAX];
     __shared__ unsigned int s_item;
     __shared__ long long lscr[NW];
-    __shared__ double dscr[NW], dscr2[NW], dscr3[NW];
+    __shared__ double dscr[2 * NW], dscr2[NW], dscr3[NW];
     __shared__ int iscr[NW];
 
     const int tid = threadIdx.x;
@@ -846,45 +934,27 @@ __global__ __launch_bounds__(NT) void k_main(Batch b, Out o, Work w,
             }
             __syncthreads();
 
-            // fold + run-merged binning over the lane's points
-            int cur[SUBMAX];
-            long long aw[SUBMAX], ay[SUBMAX];
-#pragma unroll
-            for (int s = 0; s < SUBMAX; ++s) {
-                cur[s] = -1;
-                aw[s] = 0;
-                ay[s] = 0;
-            }
-            const double *tp = w.ws_t + P.ws_off + tid;
-            const long long *wp = w.ws_w + P.ws_off + tid;
-            const long long *yp = w.ws_y + P.ws_off + tid;
-            for (int i = 0; i < P.L; ++i) {
-                const double tv = tp[(long long)i * NT];
-                const long long wv = wp[(long long)i * NT];
-                const long long yv = yp[(long long)i * NT];
-#pragma unroll
-                for (int s = 0; s < SUBMAX; ++s) {
-                    if (s < cnt) {
-                        int ind = fold_bin(tv, per[s], inv[s], bd, inv_bd);
-                        ind = min(max(ind, 0), nb[s]);
-                        if (ind != cur[s]) {
-                            if (cur[s] >= 0)
-                                flush_run(binW, binY, base[s] + cur[s], aw[s],
-                                          ay[s]);
-                            cur[s] = ind;
-                            aw[s] = wv;
-                            ay[s] = yv;
-                        } else {
-                            aw[s] += wv;
-                            ay[s] += yv;
-                        }
-                    }
+            // fold + binning over the lane's points: the lane-major layout
+            // puts the 64 lanes of a wave ~N/256 points apart in time, so
+            // their bins rarely collide and every point goes straight to
+            // the LDS histogram (padding slots carry zero weights)
+            {
+                const double *tp = w.ws_t + P.ws_off + tid;
+                const long long *wp = w.ws_w + P.ws_off + tid;
+                const long long *yp = w.ws_y + P.ws_off + tid;
+                const int L = P.L;
+#define ATG_BIN(C)                                                          \
+    case C:                                                                 \
+        bin_points<C>(tp, wp, yp, L, per, inv, nb, base, bd, inv_bd, binW,  \
+                      binY);                                                \
+        break;
+                switch (cnt) {
+                    ATG_BIN(1) ATG_BIN(2) ATG_BIN(3) ATG_BIN(4)
+                    ATG_BIN(5) ATG_BIN(6) ATG_BIN(7) ATG_BIN(8)
+                    default: break;
                 }
+#undef ATG_BIN
             }
-#pragma unroll
-            for (int s = 0; s < SUBMAX; ++s)
-                if (s < cnt && cur[s] >= 0)
-                    flush_run(binW, binY, base[s] + cur[s], aw[s], ay[s]);
             __syncthreads();
 
             int bse = 0;
@@ -1244,7 +1314,7 @@ struct atg_ctx {
     std::string err;
     void *ws = nullptr;
     size_t ws_bytes = 0;
-    atg_tuning tune{8, 4096, 2, 0};
+    atg_tuning tune{8, 3072, 3, 0};
     int num_cu = 0;
     size_t lds_max = 65536;
     size_t lds_attr_set = 65536;
