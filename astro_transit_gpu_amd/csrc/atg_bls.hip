@@ -229,7 +229,7 @@ __device__ __forceinline__ int fold_exact(double t, double P, double bd) {
 // scheduler can interleave; the next point's loads are issued before the
 // current point is folded.  Bins that a near-integer quotient makes
 // uncertain are recomputed with true divisions before the atomics.
-template <int C>
+template <int C, bool EXACT>
 __device__ __forceinline__ void bin_points(
     const double *tp, const long long *wp, const long long *yp, int L,
     const double *per, const double *inv, const int *nb, const int *base,
@@ -248,11 +248,18 @@ __device__ __forceinline__ void bin_points(
         unsigned flags = 0;
 #pragma unroll
         for (int s = 0; s < C; ++s) {
-            bool f;
-            ind[s] = fold_fast(tv, per[s], inv[s], inv_bd, f);
-            flags |= (unsigned)f << s;
+            if (EXACT) {
+                bool f;
+                ind[s] = fold_fast(tv, per[s], inv[s], inv_bd, f);
+                flags |= (unsigned)f << s;
+            } else {
+                // fast mode: FMA phase and rounded reciprocals only; a point
+                // within ~1e-15 of a bin edge may land in the neighbour bin
+                const double ph = fma(-per[s], floor(tv * inv[s]), tv);
+                ind[s] = (int)(ph * inv_bd) + 1;
+            }
         }
-        if (flags) {
+        if (EXACT && flags) {
 #pragma unroll
             for (int s = 0; s < C; ++s)
                 if ((flags >> s) & 1u) ind[s] = fold_exact(tv, per[s], bd);
@@ -943,15 +950,25 @@ __global__ __launch_bounds__(NT) void k_main(Batch b, Out o, Work w,
                 const long long *wp = w.ws_w + P.ws_off + tid;
                 const long long *yp = w.ws_y + P.ws_off + tid;
                 const int L = P.L;
-#define ATG_BIN(C)                                                          \
+#define ATG_BIN(C, E)                                                       \
     case C:                                                                 \
-        bin_points<C>(tp, wp, yp, L, per, inv, nb, base, bd, inv_bd, binW,  \
-                      binY);                                                \
+        bin_points<C, E>(tp, wp, yp, L, per, inv, nb, base, bd, inv_bd,     \
+                         binW, binY);                                       \
         break;
-                switch (cnt) {
-                    ATG_BIN(1) ATG_BIN(2) ATG_BIN(3) ATG_BIN(4)
-                    ATG_BIN(5) ATG_BIN(6) ATG_BIN(7) ATG_BIN(8)
-                    default: break;
+                if (b.mode == ATG_MODE_FAST) {
+                    switch (cnt) {
+                        ATG_BIN(1, false) ATG_BIN(2, false) ATG_BIN(3, false)
+                        ATG_BIN(4, false) ATG_BIN(5, false) ATG_BIN(6, false)
+                        ATG_BIN(7, false) ATG_BIN(8, false)
+                        default: break;
+                    }
+                } else {
+                    switch (cnt) {
+                        ATG_BIN(1, true) ATG_BIN(2, true) ATG_BIN(3, true)
+                        ATG_BIN(4, true) ATG_BIN(5, true) ATG_BIN(6, true)
+                        ATG_BIN(7, true) ATG_BIN(8, true)
+                        default: break;
+                    }
                 }
 #undef ATG_BIN
             }

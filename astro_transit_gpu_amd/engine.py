@@ -266,6 +266,24 @@ class Context:
         self._check(fn(self._h, ctypes.byref(b), ctypes.byref(o)))
         return out
 
+    def capture(self, batch: DeviceBatch, out: DeviceResult,
+                objective="likelihood", mode="parity", oversample=10):
+        """Capture one run of ``batch`` into ``out`` as a HIP graph
+        (torch.cuda.CUDAGraph).  The workspace is sized by an eager run first;
+        the captured launch sequence holds no memset or allocation."""
+        self.run(batch, objective, mode, oversample, out=out, sync=True)
+        g = torch.cuda.CUDAGraph()
+        prev = self.stream
+        cap = torch.cuda.Stream(self.device)
+        try:
+            with torch.cuda.graph(g, stream=cap):
+                self.set_stream(torch.cuda.current_stream(self.device))
+                self.run(batch, objective, mode, oversample, out=out,
+                         sync=False)
+        finally:
+            self.set_stream(prev)
+        return g
+
     def workspace_bytes(self, batch: DeviceBatch):
         b, _ = self._structs(batch, DeviceResult.empty(
             1, 1, batch.host_period_offsets, self.device), "likelihood",

@@ -75,7 +75,7 @@ def main():
     ap.add_argument("--n-lc", type=int, default=1000)
     ap.add_argument("--plain", type=int, default=20,
                     help="curves timed in the one-curve-per-call form")
-    ap.add_argument("--cpu-sample", type=int, default=8)
+    ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--skip-extras", action="store_true")
     args = ap.parse_args()

@@ -153,3 +153,43 @@ def test_segmented_bins_bit_identical(name):
     ib = int(np.argmax(c["out"]["power"]))
     for k in c["out"]:
         assert b[k][ib] == c["out"][k][ib], k
+
+
+@pytest.mark.parametrize("name", ["cfg1_likelihood", "cfg1_snr",
+                                  "hetero_dy"])
+def test_fast_mode_drift(ctx, name):
+    """Fast mode (FMA fold without the bin-edge exactness check, no
+    refinement) stays within 1e-6 relative power of astropy with the same
+    best period, and is run-to-run bit-identical (int64 bins)."""
+    c = O.load_golden([p for p in GOLDEN if p.endswith(name + ".npz")][0])
+    a = run_single(ctx, c, mode="fast")
+    b = run_single(ctx, c, mode="fast")
+    for k in ("power", "depth", "transit_time", "duration"):
+        assert np.array_equal(a[k], b[k]), k
+    rel = assert_parity(a, c["out"], name + " fast")
+    print("fast drift %.3e window agreement %.4f" % (
+        rel, window_agreement(a, c["out"])))
+
+
+def test_hipgraph_replays_match_eager(ctx):
+    """The launch sequence captured as a HIP graph gives the eager result on
+    every replay, with eager work and synchronisations in between."""
+    from astro_transit_gpu_amd import DeviceBatch, DeviceResult, synth
+    d = synth.config2(n_lc=8, seed=23, n_points=5000, n_periods=600)
+    b = DeviceBatch.from_host(d["t"], d["y"], d["dy"], d["periods"],
+                              d["durations"], device=ctx.device)
+    eager = ctx.run(b, objective="likelihood").to_host()
+    out = DeviceResult.empty(b.n_periods, b.n_lc, b.host_period_offsets,
+                             ctx.device)
+    g = ctx.capture(b, out, objective="likelihood")
+    for rep in range(4):
+        for k in ("power", "depth"):
+            getattr(out, k).fill_(-1.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        h = out.to_host()
+        for k in h:
+            assert np.array_equal(h[k], eager[k]), (rep, k)
+        ctx.run(b, objective="snr")  # unrelated eager work between replays
+        torch.cuda.synchronize()
