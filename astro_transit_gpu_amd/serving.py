@@ -170,7 +170,11 @@ class BLSServer:
         return r
 
     def _admit(self):
-        n = 0
+        """Admit pending requests in arrival order while they fit the rings,
+        then stage the whole round: per ring, the new requests' arrays are
+        packed into one pinned buffer and copied with one H2D per contiguous
+        run (a wrap splits a run)."""
+        round_ = []
         while self.pending:
             r = self.pending[0]
             nt, np_, nd = (_al(r.t.size), _al(r.periods.size),
@@ -183,21 +187,43 @@ class BLSServer:
             self.r_p.commit(sp, np_)
             self.r_d.commit(sd, nd)
             r.pt, r.pp, r.pd = st, sp, sd
-            with torch.cuda.stream(self.copy_stream):
-                def up(dst, off, a):
-                    src = torch.from_numpy(a).pin_memory()
-                    dst[off:off + a.size].copy_(src, non_blocking=True)
-                up(self.a_t, r.pt, r.t)
-                up(self.a_y, r.pt, r.y)
-                if r.dy is not None:
-                    up(self.a_dy, r.pt, r.dy)
-                up(self.a_p, r.pp, r.periods)
-                up(self.a_d, r.pd, r.durations)
             r.admitted = True
             self.pending.popleft()
-            self.admitted.append(r)
-            n += 1
-        return n
+            round_.append(r)
+        if not round_:
+            return 0
+        with torch.cuda.stream(self.copy_stream):
+            self._stage(round_, "pt", self.a_t, lambda r: r.t, np.float64)
+            self._stage(round_, "pt", self.a_y, lambda r: r.y, self.y_dtype)
+            self._stage([r for r in round_ if r.dy is not None], "pt",
+                        self.a_dy, lambda r: r.dy, self.y_dtype)
+            self._stage(round_, "pp", self.a_p, lambda r: r.periods,
+                        np.float64)
+            self._stage(round_, "pd", self.a_d, lambda r: r.durations,
+                        np.float64)
+        self.admitted.extend(round_)
+        return len(round_)
+
+    def _stage(self, reqs, attr, dst, get, dtype):
+        i = 0
+        while i < len(reqs):
+            j = i + 1
+            end = getattr(reqs[i], attr) + get(reqs[i]).size
+            while j < len(reqs) and getattr(reqs[j], attr) == end:
+                end += get(reqs[j]).size
+                j += 1
+            start = getattr(reqs[i], attr)
+            n = end - start
+            if n:
+                pin = torch.empty(n, dtype=dst.dtype, pin_memory=True)
+                host = pin.numpy()
+                o = 0
+                for r in reqs[i:j]:
+                    a = get(r)
+                    host[o:o + a.size] = a
+                    o += a.size
+                dst[start:end].copy_(pin, non_blocking=True)
+            i = j
 
     # ------------------------------------------------------------ batching
     def _take_batch(self):
@@ -276,7 +302,8 @@ class BLSServer:
         po = b["po"]
         h = {k: v.numpy() for k, v in b["host"].items()}
         for i, r in enumerate(b["reqs"]):
-            r.result = {k: h[k][po[i]:po[i + 1]].copy() for k in FIELDS}
+            # views of the batch's pinned result buffers (kept alive by them)
+            r.result = {k: h[k][po[i]:po[i + 1]] for k in FIELDS}
             r.best_index = int(b["bi"][i])
             r.status = int(b["st"][i])
             r.done = now
@@ -323,7 +350,7 @@ class BLSServer:
 
 
 def serve_benchmark(d, seconds=8.0, batched_rate=4000.0, load=0.7,
-                    objective="likelihood", seed=5, max_batch=512):
+                    objective="likelihood", seed=5, max_batch=256):
     """Poisson arrivals of the 15k-point curves of ``d`` (reused cyclically)
     at ``load`` x ``batched_rate`` requests/s for ``seconds``; reports the
     served rate and latency percentiles."""

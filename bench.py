@@ -78,6 +78,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--skip-extras", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default tiling")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (a kernel "
+                    "of known HBM traffic for counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this engine has no multi-GPU "
@@ -92,6 +97,21 @@ def main():
     objective = "likelihood"
     d = synth.config2(n_lc=args.n_lc, seed=2)
     ctx = Context(0)
+    if args.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device=ctx.device)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    from astro_transit_gpu_amd.autotune import autotune
+    t0 = time.perf_counter()
+    if args.no_autotune:
+        tuning = ctx.get_tuning()
+    else:
+        tuning, _ = autotune(ctx, d["t"], d["y"], d["dy"], d["periods"],
+                             d["durations"], objective=objective,
+                             mode="parity", use_cache=False)
+    tune_s = time.perf_counter() - t0
     batch = DeviceBatch.from_host(d["t"], d["y"], d["dy"], d["periods"],
                                   d["durations"], device=ctx.device)
     out = ctx.run(batch, objective=objective, mode="parity")
@@ -130,7 +150,8 @@ def main():
                    "durations 0.04-0.20 d, likelihood, parity mode",
                    "n_lc": args.n_lc, "n_points": 15000, "n_periods": 5000,
                    "n_durations": 5, "objective": objective,
-                   "mode": "parity"},
+                   "mode": "parity", "tiling": tuning,
+                   "autotune_s": round(tune_s, 2)},
         "roofline": {
             "bound": "valu", "achieved": round(ach_valu / 1e12, 3),
             "peak": round(PEAK_VALU / 1e12, 3), "unit": "Tlane-op/s",
