@@ -172,3 +172,20 @@ def config3(n_lc=16384, seed=3, n_lo=1000, n_hi=20000, p_lo=1000,
         out["durations"].append(durs)
         out["periods"].append(capped_grid(t, durs, p_lo, p_hi))
     return out
+
+
+def config5_pool(n=128, seed=5, n_lo=1000, n_hi=200000, base_lo=10.0,
+                 base_hi=360.0, p_cap=20000):
+    """Pool of serving requests: N log-uniform 1k-200k, baseline uniform
+    10-360 d, sorted uniform times with a random gap, optional injected box,
+    durations 0.04-0.20 d, autoperiod grid capped at ``p_cap`` periods (the
+    uncapped grid reaches millions of periods at 360 d)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        t, y, dy = ragged_curve(rng, n_lo, n_hi, base_lo, base_hi)
+        durs = DURATIONS_5.copy()
+        per = capped_grid(t, durs, 1000, p_cap)
+        out.append((t, y.astype(np.float32), dy.astype(np.float32), per,
+                    durs))
+    return out

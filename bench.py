@@ -67,6 +67,71 @@ def cpu_baseline(d, n_sample, objective):
                 "oracle (OpenMP over periods), %.1f s" % (n_sample, el))
 
 
+def ragged_leg(ctx, n_lc, objective):
+    """Ragged CSR batch: N log-uniform 1k-20k, baselines 10-27 d, per-curve
+    autoperiod grids clipped to 1k-20k periods, 5-8 durations; one launch."""
+    import torch
+    from astro_transit_gpu_amd import DeviceBatch, synth
+    t0 = time.perf_counter()
+    d = synth.config3(n_lc=n_lc, seed=3)
+    gen = time.perf_counter() - t0
+    b = DeviceBatch.from_host(d["t"], d["y"], d["dy"], d["periods"],
+                              d["durations"], device=ctx.device)
+    out = ctx.run(b, objective=objective)
+    torch.cuda.synchronize()
+    ctx.timing(True, True)
+    s = time.perf_counter()
+    ctx.run(b, objective=objective, out=out, sync=False)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - s
+    _, kms = ctx.timing(False, True)
+    folds = sum(np.size(t) * np.size(p) for t, p in zip(d["t"], d["periods"]))
+    st = out.status.cpu().numpy()
+    return {"value": round(n_lc / el, 1), "unit": "LC/s", "curves": n_lc,
+            "points": int(sum(np.size(t) for t in d["t"])),
+            "periods": int(sum(np.size(p) for p in d["periods"])),
+            "folds_per_s": float("%.4g" % (folds / el)),
+            "kernel_ms": round(kms, 2), "statuses_ok": int((st == 0).sum()),
+            "generation_s": round(gen, 1)}
+
+
+def serve5_leg(seconds, objective, rate=2000.0, pool=96, budget=16e9):
+    """Serving stream: Poisson arrivals at ``rate`` req/s for ``seconds``
+    from a pool of requests with N 1k-200k, baselines 10-360 d and capped
+    autoperiod grids; admission by an HBM budget.  The offered load exceeds
+    the device, so this reports sustained throughput and latency at
+    saturation."""
+    from astro_transit_gpu_amd import synth
+    from astro_transit_gpu_amd.serving import BLSServer
+    reqs = synth.config5_pool(pool)
+    srv = BLSServer(objective=objective, hbm_budget_bytes=budget,
+                    typical_n=40000, typical_p=20000, max_batch=64)
+    rng = np.random.default_rng(7)
+    n_req = int(rate * seconds)
+    arr = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    handles = []
+    start = time.perf_counter()
+    i = 0
+    while i < n_req or srv.pending or srv.admitted or srv.inflight:
+        now = time.perf_counter() - start
+        while i < n_req and arr[i] <= now:
+            handles.append(srv.submit(*reqs[i % pool],
+                                      arrival=start + arr[i]))
+            i += 1
+        if not srv.poll():
+            time.sleep(0.0005)
+    el = time.perf_counter() - start
+    lat = np.array([h.latency for h in handles]) * 1e3
+    pts = sum(np.size(h.t) for h in handles)
+    return {"value": round(len(handles) / el, 1), "unit": "LC/s",
+            "offered_rate": rate, "requests": len(handles),
+            "seconds": round(el, 2), "mean_points": int(pts / len(handles)),
+            "p50_ms": round(float(np.percentile(lat, 50)), 1),
+            "p99_ms": round(float(np.percentile(lat, 99)), 1),
+            "hbm_budget_GB": budget / 1e9, "period_cap": 20000,
+            "statuses_ok": int(sum(h.status == 0 for h in handles))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +142,9 @@ def main():
                     help="curves timed in the one-curve-per-call form")
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--ragged-lc", type=int, default=2048,
+                    help="curves of the ragged-batch leg (16,384 = full)")
+    ap.add_argument("--serve5-seconds", type=float, default=3.0)
     ap.add_argument("--skip-extras", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tiling")
@@ -192,6 +260,8 @@ def main():
                                              objective=objective)
         except ImportError:
             line["served"] = None
+        line["ragged"] = ragged_leg(ctx, args.ragged_lc, objective)
+        line["served_stream"] = serve5_leg(args.serve5_seconds, objective)
         line["cpu_baseline"] = cpu_baseline(d, args.cpu_sample, objective)
     print(json.dumps(line), flush=True)
 
