@@ -46,6 +46,7 @@ constexpr int NT = 256;      // threads per workgroup = 4 wave64
 constexpr int NW = NT / 64;  // waves per workgroup
 constexpr int SUBMAX = 8;    // periods folded together in one pass
 constexpr int DMAX = 64;     // durations per light curve
+constexpr int NQ = 8;        // work queues = XCDs of an MI355X
 
 struct LcParams {
     double t_ref;     // min(t)
@@ -88,7 +89,9 @@ struct Out {
 struct Work {
     LcParams *prm;
     int *lc_flags;        // per-curve flags raised by k_main (LDS overflow)
-    long long *item_off;  // [n_lc + 1]
+    long long *item_off;  // [n_lc] first item of the curve in its queue
+    long long *qtot;      // [NQ] items per queue
+    unsigned int *qnext;  // [NQ] queue heads
     unsigned int *counter;
     double *ws_t;         // lane-major t'
     long long *ws_w;      // lane-major quantized ivar
@@ -472,22 +475,34 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w) {
 
 // ----------------------------------------------------------------- k_items
 
+// Work items of curve c go to queue c mod NQ.  Persistent workgroups are
+// dispatched round-robin over the XCDs (workgroup i on XCD i mod NQ) and
+// drain the queue of their own XCD first, so the points of a curve are
+// fetched into one XCD's L2 instead of all eight.
 __global__ __launch_bounds__(NT) void k_items(long long n_lc,
                                               const long long *per_off,
                                               int per_item, Work w) {
     __shared__ long long lscr[NW];
-    long long carry = 0;
-    for (long long base = 0; base < n_lc; base += NT) {
-        long long i = base + threadIdx.x;
-        long long c = 0;
-        if (i < n_lc) c = (per_off[i + 1] - per_off[i] + per_item - 1) / per_item;
-        long long tot;
-        long long e = block_excl_scan(c, lscr, &tot);
-        if (i < n_lc) w.item_off[i] = carry + e;
-        carry += tot;
+    for (int q = 0; q < NQ; ++q) {
+        const long long nq = (n_lc - q + NQ - 1) / NQ;
+        long long carry = 0;
+        for (long long base = 0; base < nq; base += NT) {
+            const long long j = base + threadIdx.x;
+            const long long lc = q + NQ * j;
+            long long c = 0;
+            if (j < nq)
+                c = (per_off[lc + 1] - per_off[lc] + per_item - 1) / per_item;
+            long long tot;
+            const long long e = block_excl_scan(c, lscr, &tot);
+            if (j < nq) w.item_off[lc] = carry + e;
+            carry += tot;
+        }
+        if (threadIdx.x == 0) {
+            w.qtot[q] = carry;
+            w.qnext[q] = 0u;
+        }
     }
     if (threadIdx.x == 0) {
-        w.item_off[n_lc] = carry;
         *w.counter = 0u;
         *w.cand_count = 0u;
         *w.cand_next = 0u;
@@ -854,29 +869,46 @@ __global__ __launch_bounds__(NT) void k_main(Batch b, Out o, Work w,
     long long *binY = binW + lds_bins;
     __shared__ int s_m[DMAX];
     __shared__ unsigned int s_item;
+    __shared__ int s_queue;
     __shared__ long long lscr[NW];
     __shared__ double dscr[2 * NW], dscr2[NW], dscr3[NW];
     __shared__ int iscr[NW];
 
     const int tid = threadIdx.x;
     const long long n_lc = b.n_lc;
-    const unsigned int total = (unsigned int)w.item_off[n_lc];
+    const int home = blockIdx.x % NQ;
+    int queue = home;
     for (;;) {
         __syncthreads();
-        if (tid == 0) s_item = atomicAdd(w.counter, 1u);
+        if (tid == 0) {
+            // own XCD's queue first, then steal from the others in turn
+            int qq = queue, left = NQ;
+            unsigned int it = 0xffffffffu;
+            while (left > 0) {
+                it = atomicAdd(&w.qnext[qq], 1u);
+                if ((long long)it < w.qtot[qq]) break;
+                it = 0xffffffffu;
+                qq = (qq + 1) % NQ;
+                --left;
+            }
+            s_item = it;
+            s_queue = qq;
+        }
         __syncthreads();
         const unsigned int item = s_item;
-        if (item >= total) break;
-        // curve of the item: last lc with item_off[lc] <= item
-        long long lo = 0, hi = n_lc;
+        if (item == 0xffffffffu) break;
+        queue = s_queue;
+        // curve of the item: last curve of the queue whose first item <= item
+        const long long nq = (n_lc - queue + NQ - 1) / NQ;
+        long long lo = 0, hi = nq;
         while (hi - lo > 1) {
             long long mid = (lo + hi) >> 1;
-            if ((unsigned long long)w.item_off[mid] <= item) lo = mid;
-            else hi = mid;
+            if ((unsigned long long)w.item_off[queue + NQ * mid] <= item)
+                lo = mid;
+            else
+                hi = mid;
         }
-        while (lo + 1 < n_lc && (unsigned long long)w.item_off[lo + 1] <= item)
-            ++lo;
-        const int lc = (int)lo;
+        const int lc = (int)(queue + NQ * lo);
         const LcParams P = w.prm[lc];
         const long long pbeg = b.per_off[lc] +
                                (long long)(item - w.item_off[lc]) * per_item;
@@ -1357,7 +1389,8 @@ int hip_fail(atg_ctx *c, hipError_t e, const char *what) {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct Layout {
-    size_t prm, flags, items, counter, cand_lc, cand_p, t, w, y, total;
+    size_t prm, flags, items, queues, counter, cand_lc, cand_p, t, w, y,
+        total;
 };
 
 Layout layout_of(const atg_bls_batch *b) {
@@ -1368,6 +1401,7 @@ Layout layout_of(const atg_bls_batch *b) {
     L.prm = o;     o = align256(o + nlc * sizeof(LcParams));
     L.flags = o;   o = align256(o + nlc * sizeof(int));
     L.items = o;   o = align256(o + (nlc + 1) * sizeof(long long));
+    L.queues = o;  o = align256(o + NQ * (sizeof(long long) + sizeof(int)));
     L.counter = o; o = align256(o + 3 * sizeof(unsigned int));
     L.cand_lc = o; o = align256(o + nlc * CAP * sizeof(int));
     L.cand_p = o;  o = align256(o + nlc * CAP * sizeof(long long));
@@ -1552,6 +1586,8 @@ int atg_bls_run(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o) {
     w.lc_flags = (int *)(base + L.flags);
     w.item_off = (long long *)(base + L.items);
     w.counter = (unsigned int *)(base + L.counter);
+    w.qtot = (long long *)(base + L.queues);
+    w.qnext = (unsigned int *)(base + L.queues + NQ * sizeof(long long));
     w.ws_t = (double *)(base + L.t);
     w.ws_w = (long long *)(base + L.w);
     w.ws_y = (long long *)(base + L.y);

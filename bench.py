@@ -54,6 +54,20 @@ def algorithmic_work(n_points, periods, durations, oversample=10):
                 sum_bins=int(nb.sum()))
 
 
+def measured_traffic(n_lc):
+    """HBM bytes per k_main launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-kernel calibrated), scaled to n_lc curves."""
+    path = os.path.join(ROOT, "profiles",
+                        "k_main_hbm_traffic_1000x15k_5000p_5d.md")
+    try:
+        kv = dict(line.strip().split("=", 1) for line in open(path)
+                  if "=" in line and line.strip().split("=", 1)[0] in
+                  ("n_lc", "traffic_bytes"))
+        return int(int(kv["traffic_bytes"]) * n_lc / int(kv["n_lc"]))
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(d, n_sample, objective):
     from oracle import oracle
     cores = min(len(os.sched_getaffinity(0)), 16)
@@ -223,14 +237,16 @@ def main():
         "roofline": {
             "bound": "valu", "achieved": round(ach_valu / 1e12, 3),
             "peak": round(PEAK_VALU / 1e12, 3), "unit": "Tlane-op/s",
-            "frac": round(ach_valu / PEAK_VALU, 4), "traffic": None,
+            "frac": round(ach_valu / PEAK_VALU, 4),
+            "traffic": measured_traffic(args.n_lc),
             "kernel": "k_main", "kernel_ms": round(kms, 3),
             "valu_lane_ops_per_launch": per_launch_valu,
             "lds": {"achieved_GBps": round(ach_lds / 1e9, 1),
                     "peak_GBps": PEAK_LDS / 1e9,
                     "frac": round(ach_lds / PEAK_LDS, 4),
                     "bytes_per_launch": per_launch_lds},
-            "hbm_algorithmic_bytes_per_launch": hbm_in + hbm_out,
+            "hbm_algorithmic_bytes_per_launch": args.n_lc * 15000 * 24 +
+            hbm_out,
         },
     }
     if not args.skip_extras:
