@@ -63,6 +63,11 @@ struct LcParams {
     int n;            // points
     int status;       // ATG_LC_*
     int n_dur;
+    double seq_y;     // parity mode: the oracle's totals, summed in FP64
+    double seq_iv;    // in point order (sum of y*ivar, sum of ivar)
+    int sorted;       // t non-decreasing (enables the ranged phase tiles)
+    int lc_self;      // index of the curve
+    int natural;      // natural-order copies of the points were written
 };
 
 struct Batch {
@@ -96,6 +101,9 @@ struct Work {
     double *ws_t;         // lane-major t'
     long long *ws_w;      // lane-major quantized ivar
     long long *ws_y;      // lane-major quantized ivar * (y - median)
+    double *nat_t;        // the same three arrays in point order, written
+    long long *nat_w;     // only for time-sorted curves with periods whose
+    long long *nat_y;     // bins exceed the LDS histogram
     unsigned int *cand_count;  // periods queued for refinement
     unsigned int *cand_next;   // refinement queue head
     int *cand_lc;         // [n_lc * CAP] curve of each queued period
@@ -227,6 +235,50 @@ __device__ __forceinline__ int fold_exact(double t, double P, double bd) {
     return (int)trunc(ph / bd) + 1;
 }
 
+// Cycle index floor(t/P) of the oracle's fold, with the same near-integer
+// fallback as fold_bin, as a double.
+__device__ __forceinline__ double fold_cycle(double t, double P,
+                                            double invP) {
+    const double r = t * invP;
+    double q = floor(r);
+    const double fr = r - q;
+    const double tol = r * 0x1p-50;
+    if (fr <= tol || fr >= 1.0 - tol) q = floor(t / P);
+    return q;
+}
+
+// Bin of t within cycle q (q = fold_cycle(t)), clamped to [0, nb]:
+// equal to min(max(fold_bin(t), 0), nb).
+__device__ __forceinline__ int fold_bin_in(double t, double P, double q,
+                                           double bd, double inv_bd, int nb) {
+    const double ph = t - P * q;
+    const double x = ph * inv_bd;
+    double xi = trunc(x);
+    const double fx = x - xi;
+    const double tolx = fabs(x) * 0x1p-50;
+    if (fx <= tolx || fx >= 1.0 - tolx) xi = trunc(ph / bd);
+    return min(max((int)xi + 1, 0), nb);
+}
+
+// For time-sorted points the key (cycle, bin) of the oracle's fold is
+// non-decreasing in the point index: floor(t/P) is monotone in t and, for a
+// fixed cycle, so are t - P*q, the division by bd and the truncation.  The
+// first index whose key is >= (c, s) is therefore a binary search.
+__device__ int lower_key(const double *tn, int n, double P, double invP,
+                         double bd, double inv_bd, int nb, double c, int s) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const double t = tn[mid];
+        const double q = fold_cycle(t, P, invP);
+        const bool less =
+            q < c || (q == c && fold_bin_in(t, P, q, bd, inv_bd, nb) < s);
+        if (less) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // Bins every point of the lane for C periods at once.  C is a compile-time
 // constant so that the C fold chains are independent straight-line code the
 // scheduler can interleave; the next point's loads are issued before the
@@ -332,7 +384,22 @@ __device__ void select2(const void *y, int dt, long long off, int n, int k0,
     *v1 = ikey(pre1);
 }
 
-__global__ __launch_bounds__(NT) void k_prep(Batch b, Work w) {
+// Fixed-point weight and weight*flux of point i (shared by both layouts).
+__device__ __forceinline__ void quantize(const Batch &b, long long i,
+                                         double med, double wscale,
+                                         double yscale, long long &wq,
+                                         long long &yq) {
+    double iv = 1.0;
+    if (b.dy) {
+        const double d = load_val(b.dy, b.dy_dt, i);
+        iv = 1.0 / (d * d);
+    }
+    const double yc = load_val(b.y, b.y_dt, i) - med;
+    wq = llrint(iv * wscale);
+    yq = llrint((yc * iv) * yscale);
+}
+
+__global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     __shared__ unsigned int hist[512];
     __shared__ int sel[4];
     __shared__ unsigned int uscr[NW];
@@ -352,6 +419,7 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w) {
     memset(&P, 0, sizeof(P));
     P.n = n;
     P.n_dur = (int)nd;
+    P.lc_self = lc;
     P.ws_off = off + (long long)lc * NT;
     P.L = (n + NT - 1) / NT;
     int status = ATG_LC_OK;
@@ -359,12 +427,14 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w) {
     else if (nd > DMAX) status = ATG_LC_TOO_MANY_DURATIONS;
 
     // grid checks in the oracle's order: periods, then durations
-    double pmin = INFINITY, dmin = INFINITY, dmax = -INFINITY;
+    double pmin = INFINITY, pmax = -INFINITY, dmin = INFINITY,
+           dmax = -INFINITY;
     int bad_grid = 0;
     for (long long i = tid; i < np; i += NT) {
         double p = b.periods[po + i];
         if (!(p >= DBL_EPSILON)) bad_grid |= 1;
         pmin = fmin(pmin, p);
+        pmax = fmax(pmax, p);
     }
     for (long long i = tid; i < nd; i += NT) {
         double d = b.durs[dof + i];
@@ -373,6 +443,7 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w) {
         dmax = fmax(dmax, d);
     }
     pmin = block_reduce(pmin, MinOp(), dscr);
+    pmax = block_reduce(pmax, MaxOp(), dscr);
     dmin = block_reduce(dmin, MinOp(), dscr);
     dmax = block_reduce(dmax, MaxOp(), dscr);
     bad_grid = block_reduce(bad_grid, OrOp(), iscr);
@@ -381,13 +452,14 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w) {
         else if ((bad_grid & 2) || dmax > pmin) status = ATG_LC_BAD_DURATION;
     }
 
-    // pass A: min(t), NaN and dy checks
+    // pass A: min(t), NaN and dy checks, time order
     double tmin = INFINITY;
     int bad = 0;
     for (int i = tid; i < n; i += NT) {
         double tv = b.t[off + i];
         double yv = load_val(b.y, b.y_dt, off + i);
         if (isnan(tv) || isnan(yv)) bad |= 1;
+        if (i > 0 && !(tv >= b.t[off + i - 1])) bad |= 4;
         tmin = fmin(tmin, tv);
         if (b.dy) {
             double d = load_val(b.dy, b.dy_dt, off + i);
@@ -402,6 +474,7 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w) {
     }
     P.t_ref = tmin;
     P.status = status;
+    P.sorted = (bad & 4) ? 0 : 1;
     if (status != ATG_LC_OK) {
         if (tid == 0) w.prm[lc] = P;
         return;
@@ -413,6 +486,30 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w) {
             uscr);
     const double med = (n & 1) ? v0 : (v0 + v1) * 0.5;
     P.y_med = med;
+
+    // parity mode: the oracle's FP64 totals in its order (one wave walks
+    // the points; the refinement kernel uses them)
+    if (b.mode == ATG_MODE_PARITY && tid < 64) {
+        double sy = 0.0, si = 0.0;
+        for (int base = 0; base < n; base += 64) {
+            const int j = base + tid;
+            double yv = 0.0, iv = 1.0;
+            if (j < n) {
+                if (b.dy) {
+                    const double d = load_val(b.dy, b.dy_dt, off + j);
+                    iv = 1.0 / (d * d);
+                }
+                yv = (load_val(b.y, b.y_dt, off + j) - med) * iv;
+            }
+            const int cnt = min(64, n - base);
+            for (int k = 0; k < cnt; ++k) {
+                sy = sy + readlane_d(yv, k);
+                si = si + readlane_d(iv, k);
+            }
+        }
+        P.seq_y = sy;
+        P.seq_iv = si;
+    }
 
     // pass B: magnitudes for the fixed-point scales
     double sw_sum = 0.0, sy_sum = 0.0;
@@ -447,16 +544,7 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w) {
         int s = src < n ? src : n - 1;
         double tp = b.t[off + s] - tmin;
         long long wq = 0, yq = 0;
-        if (src < n) {
-            double iv = 1.0;
-            if (b.dy) {
-                double d = load_val(b.dy, b.dy_dt, off + s);
-                iv = 1.0 / (d * d);
-            }
-            double yc = load_val(b.y, b.y_dt, off + s) - med;
-            wq = llrint(iv * wscale);
-            yq = llrint((yc * iv) * yscale);
-        }
+        if (src < n) quantize(b, off + s, med, wscale, yscale, wq, yq);
         long long slot = P.ws_off + (long long)i * NT + tid;
         w.ws_t[slot] = tp;
         w.ws_w[slot] = wq;
@@ -466,6 +554,19 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w) {
     }
     wq_sum = block_reduce(wq_sum, AddOp<long long>(), lscr);
     yq_sum = block_reduce(yq_sum, AddOp<long long>(), lscr);
+    // pass D: point-order copies for the ranged phase tiles of periods
+    // whose bins exceed the LDS histogram (time-sorted curves only)
+    P.natural = 0;
+    if (P.sorted && (double)lds_bins < ceil(pmax / P.bd) + b.oversample + 1) {
+        P.natural = 1;
+        for (int i = tid; i < n; i += NT) {
+            long long wq, yq;
+            quantize(b, off + i, med, wscale, yscale, wq, yq);
+            w.nat_t[off + i] = b.t[off + i] - tmin;
+            w.nat_w[off + i] = wq;
+            w.nat_y[off + i] = yq;
+        }
+    }
     P.Wq_tot = wq_sum;
     P.Yq_tot = yq_sum;
     P.W_tot = (double)wq_sum * P.w_unit;
@@ -698,24 +799,44 @@ __device__ __forceinline__ void seg_add(int j, int s0, int hi, long long wv,
 }
 
 // One period whose n_bins + 1 exceeds the LDS histogram: the start
-// positions are processed in segments of S = lds_bins - mmax - 1; each
-// segment bins every point again, keeps bins [s0, s0 + S + mmax] in LDS and
-// carries the (exact, int64) sum of the bins below s0.
+// positions are processed in segments of S start positions; segment k keeps
+// bins [s0, s0 + S + mmax] in LDS on top of the exact int64 sum of all bins
+// below s0 (the carried prefix).
+//
+// Ranged form (time-sorted curves): the points of cycle c whose bin lies in
+// [s0, hi] are the index range [lower_key(c, s0), lower_key(c, hi + 1)), so
+// a segment reads only its own points, all 256 threads stride each range in
+// point order, and the carried prefix is the previous segment's scan value
+// at its last start position.  The range table (Q+1 cycles) takes the tail
+// of the histogram.  Otherwise every segment folds every point again and
+// sums the ones below s0.
 __device__ void big_period(const Batch &b, const Out &o, const Work &w,
                            const LcParams &P, long long p, int nb,
                            int lds_bins, long long *binW, long long *binY,
                            const int *s_m, int nd, int mmax, long long *lscr,
                            double *dscr, int *iscr, double *dscr2,
-                           double *dscr3) {
+                           double *dscr3, long long *s_carry) {
     const int tid = threadIdx.x, os = b.oversample;
     const double period = b.periods[p], invP = 1.0 / period;
     const double bd = P.bd, inv_bd = 1.0 / bd;
-    const int S = lds_bins - mmax - 1;
     const int nval = nb + 1;
     const bool like = b.objective == ATG_OBJ_LIKELIHOOD;
     const double Wt = P.W_tot, Yt = P.Y_tot;
     const double ytn = Yt / Wt, epsn = DBL_EPSILON / Wt;
     const double fw = P.w_unit / Wt, fy = P.y_unit / Wt;
+    // ranged form: cycles 0..Q of the sorted points
+    const double *tn = w.nat_t + b.lc_off[P.lc_self];
+    int ncyc = 0;
+    if (P.natural)
+        ncyc = (int)fold_cycle(tn[P.n - 1], period, invP) + 1;
+    const bool ranged = P.natural && ncyc <= (1 << 20) &&
+                        lds_bins - ncyc - mmax - 1 >= 64;
+    const int S = ranged ? lds_bins - ncyc - mmax - 1 : lds_bins - mmax - 1;
+    int *rng = (int *)(binW + (lds_bins - ncyc));  // [2c], [2c+1]
+    int *erng = (int *)(binY + (lds_bins - ncyc));
+    const long long *wn = w.nat_w + b.lc_off[P.lc_self];
+    const long long *yn = w.nat_y + b.lc_off[P.lc_self];
+    const int perm = (tid & 63) * NW + (tid >> 6);
     Score best;
     double bwin = 0.0, byin = 0.0;
     const double *tp = w.ws_t + P.ws_off + tid;
@@ -723,29 +844,96 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
     const long long *yp = w.ws_y + P.ws_off + tid;
     double *Cw = (double *)binW;
     double *Cy = (double *)binY;
+    if (tid == 0) {
+        s_carry[0] = 0;
+        s_carry[1] = 0;
+    }
+    __syncthreads();
     for (int s0 = 0; s0 < nb; s0 += S) {
         const int hi = min(s0 + S + mmax, nb), len = hi - s0 + 1;
+        long long cw = 0, cy = 0;
+        if (ranged) {
+            cw = s_carry[0];
+            cy = s_carry[1];
+        }
         for (int i = tid; i < len; i += NT) {
             binW[i] = 0;
             binY[i] = 0;
         }
-        __syncthreads();
-        long long cw = 0, cy = 0;
-        for (int i = 0; i < P.L; ++i) {
-            const long long wv = wp[(long long)i * NT];
-            const long long yv = yp[(long long)i * NT];
-            if ((wv | yv) == 0) continue;
-            const double tv = tp[(long long)i * NT];
-            const int r = min(max(fold_bin(tv, period, invP, bd, inv_bd), 0),
-                              nb);
-            int j1, j2;
-            pad_targets(r, nb, os, j1, j2);
-            seg_add(j1, s0, hi, wv, yv, cw, cy, binW, binY);
-            seg_add(j2, s0, hi, wv, yv, cw, cy, binW, binY);
+        const bool extra = hi >= nb - os;  // padding copies land here
+        if (ranged) {
+            const int ntask = (extra ? 4 : 2) * ncyc;
+            for (int e = tid; e < ntask; e += NT) {
+                const int c = (e >> 1) % ncyc;
+                const bool ex = e >= 2 * ncyc;
+                const int sb = ex ? ((e & 1) ? os + 1 : 1)
+                                  : ((e & 1) ? hi + 1 : s0);
+                const int v = lower_key(tn, P.n, period, invP, bd, inv_bd,
+                                        nb, (double)c, sb);
+                (ex ? erng : rng)[2 * c + (e & 1)] = v;
+            }
         }
         __syncthreads();
-        cw = block_reduce(cw, AddOp<long long>(), lscr);
-        cy = block_reduce(cy, AddOp<long long>(), lscr);
+        if (ranged) {
+            // direct destinations: bins [s0, hi] of every cycle
+            for (int c = 0; c < ncyc; ++c) {
+                const int e = rng[2 * c + 1];
+                const double qc = (double)c;
+                for (int j = rng[2 * c] + perm; j < e; j += NT) {
+                    const long long wv = wn[j], yv = yn[j];
+                    if ((wv | yv) == 0) continue;
+                    const int r = fold_bin_in(tn[j], period, qc, bd, inv_bd,
+                                              nb);
+                    if ((r < nb - os || r == nb) && r >= s0 && r <= hi) {
+                        __hip_atomic_fetch_add(&binW[r - s0], wv,
+                                               __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __hip_atomic_fetch_add(&binY[r - s0], yv,
+                                               __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                }
+            }
+            // padding copies: bins 1..os of every cycle to nb-os..nb-1
+            if (extra) {
+                for (int c = 0; c < ncyc; ++c) {
+                    const int e = erng[2 * c + 1];
+                    const double qc = (double)c;
+                    for (int j = erng[2 * c] + perm; j < e; j += NT) {
+                        const long long wv = wn[j], yv = yn[j];
+                        if ((wv | yv) == 0) continue;
+                        const int r = fold_bin_in(tn[j], period, qc, bd,
+                                                  inv_bd, nb);
+                        const int d = r + nb - os - 1;
+                        if (r >= 1 && r <= os && d >= s0 && d <= hi) {
+                            __hip_atomic_fetch_add(
+                                &binW[d - s0], wv, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_WORKGROUP);
+                            __hip_atomic_fetch_add(
+                                &binY[d - s0], yv, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        } else {
+            for (int i = 0; i < P.L; ++i) {
+                const long long wv = wp[(long long)i * NT];
+                const long long yv = yp[(long long)i * NT];
+                if ((wv | yv) == 0) continue;
+                const double tv = tp[(long long)i * NT];
+                const int r = min(max(fold_bin(tv, period, invP, bd, inv_bd),
+                                      0), nb);
+                int j1, j2;
+                pad_targets(r, nb, os, j1, j2);
+                seg_add(j1, s0, hi, wv, yv, cw, cy, binW, binY);
+                seg_add(j2, s0, hi, wv, yv, cw, cy, binW, binY);
+            }
+            __syncthreads();
+            cw = block_reduce(cw, AddOp<long long>(), lscr);
+            cy = block_reduce(cy, AddOp<long long>(), lscr);
+        }
         const int seg = (len + NT - 1) / NT;
         const int b0 = min(tid * seg, len), b1 = min(b0 + seg, len);
         long long sw = 0, sy = 0;
@@ -759,6 +947,10 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
         for (int i = b0; i < b1; ++i) {
             ow += binW[i];
             oy += binY[i];
+            if (i == S - 1) {  // prefix of the next segment's first bin
+                s_carry[0] = ow;
+                s_carry[1] = oy;
+            }
             Cw[i] = (double)ow * fw;
             Cy[i] = (double)oy * fy;
         }
@@ -873,6 +1065,7 @@ __global__ __launch_bounds__(NT) void k_main(Batch b, Out o, Work w,
     __shared__ long long lscr[NW];
     __shared__ double dscr[2 * NW], dscr2[NW], dscr3[NW];
     __shared__ int iscr[NW];
+    __shared__ long long s_carry[2];
 
     const int tid = threadIdx.x;
     const long long n_lc = b.n_lc;
@@ -959,7 +1152,8 @@ __global__ __launch_bounds__(NT) void k_main(Batch b, Out o, Work w,
                 const int nbp = (int)ceil(b.periods[p] / bd) + os;
                 if (nbp - os > os && lds_bins - mmax - 1 >= 1) {
                     big_period(b, o, w, P, p, nbp, lds_bins, binW, binY, s_m,
-                               nd, mmax, lscr, dscr, iscr, dscr2, dscr3);
+                               nd, mmax, lscr, dscr, iscr, dscr2, dscr3,
+                               s_carry);
                 } else if (tid == 0) {
                     write_nan(o, p);
                     atomicOr(&w.lc_flags[lc], 1);
@@ -1105,15 +1299,35 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
     for (int k = 0; k < nd; ++k) mmax = max(mmax, s_m[k]);
     // in LDS: raw bins, then the oracle's sequential padding; otherwise
     // segments of start positions with the padded destinations of each
-    // point and the exact FP64 prefix carried from segment to segment
+    // point and the exact FP64 prefix carried from segment to segment.
+    // Segments of time-sorted curves walk only their own points: the
+    // ranges of every cycle (as in k_main's phase tiles), then the ranges
+    // of bins 1..os whose copies land in the segment.  Each bin still
+    // receives its points in point order.
     const bool fits = nb + 1 <= lds_bins;
-    if (!fits && !(nb - os > os && lds_bins - mmax - 1 >= 1)) continue;
-    const int S = fits ? nb + 1 : lds_bins - mmax - 1;
+    const double *tn = w.nat_t + off;
+    int ncyc = 0;
+    if (!fits && P.natural)
+        ncyc = (int)fold_cycle(tn[n - 1], period, invP) + 1;
+    const int T = 2 * ncyc + 2;  // tail slots of my[] for the range table
+    const bool ranged = !fits && P.natural && nb - os > os &&
+                        ncyc <= (1 << 20) && lds_bins - T - mmax - 1 >= 64;
+    if (!fits && !ranged && !(nb - os > os && lds_bins - mmax - 1 >= 1))
+        continue;
+    const int S = fits ? nb + 1
+                       : (ranged ? lds_bins - T - mmax - 1
+                                 : lds_bins - mmax - 1);
+    int *lo_t = (int *)(my + (lds_bins - T));  // [2*ncyc] range starts
+    int *pre_t = lo_t + 2 * ncyc;               // [2*ncyc+1] item prefix
     const bool like = b.objective == ATG_OBJ_LIKELIHOOD;
     const int nval = nb + 1;
-    double sum_y = 0.0, sum_iv = 0.0, cprev_y = 0.0, cprev_i = 0.0;
+    double cprev_y = 0.0, cprev_i = 0.0;
     double bo = -INFINITY, byin = 0.0, bivin = 0.0;
     int bi = INT_MAX;
+    if (lane == 0) {
+        s_tot[0] = P.seq_y;
+        s_tot[1] = P.seq_iv;
+    }
     for (int s0 = 0; s0 < nb; s0 += S) {
         const int hi = fits ? nb : min(s0 + S + mmax, nb);
         const int len = hi - s0 + 1;
@@ -1121,9 +1335,97 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
             my[i] = 0.0;
             mi[i] = 0.0;
         }
-        __syncthreads();
         double ay = 0.0, ai = 0.0;
         int cur = -1;
+        if (ranged) {
+            const bool extra = hi >= nb - os;
+            for (int e = lane; e < 4 * ncyc; e += 64) {
+                const int k = e >> 1, c = k % ncyc;
+                int v = 0;
+                if (k < ncyc)
+                    v = lower_key(tn, n, period, invP, bd, inv_bd, nb,
+                                  (double)c, (e & 1) ? hi + 1 : s0);
+                else if (extra)
+                    v = lower_key(tn, n, period, invP, bd, inv_bd, nb,
+                                  (double)c, (e & 1) ? os + 1 : 1);
+                if (e & 1) pre_t[k] = v;  // range end, until the prefix
+                else lo_t[k] = v;
+            }
+            __syncthreads();
+            if (lane == 0) {
+                int acc = 0;
+                for (int k = 0; k < 2 * ncyc; ++k) {
+                    const int l = pre_t[k] - lo_t[k];
+                    pre_t[k] = acc;
+                    acc += l;
+                }
+                pre_t[2 * ncyc] = acc;
+            }
+            __syncthreads();
+            const int total = pre_t[2 * ncyc];
+            // item -> (range, point); the next chunk loads while the wave
+            // walks the current one
+            int nk = 0, nj = -1;
+            double t_raw = 0.0, y_raw = 0.0, d_raw = 1.0;
+            auto locate = [&](int item, int &k, int &j) {
+                int a = 0, z = 2 * ncyc;  // last k with pre_t[k] <= item
+                while (z - a > 1) {
+                    const int mid = (a + z) >> 1;
+                    if (pre_t[mid] <= item) a = mid;
+                    else z = mid;
+                }
+                k = a;
+                j = lo_t[a] + (item - pre_t[a]);
+            };
+            if (lane < total) {
+                locate(lane, nk, nj);
+                t_raw = b.t[off + nj];
+                y_raw = load_val(b.y, b.y_dt, off + nj);
+                if (b.dy) d_raw = load_val(b.dy, b.dy_dt, off + nj);
+            }
+            for (int cb = 0; cb < total; cb += 64) {
+                const int k = nk;
+                const bool valid = cb + lane < total;
+                const double tp = t_raw - P.t_ref;
+                const double iv = b.dy ? 1.0 / (d_raw * d_raw) : 1.0;
+                const double yv = (y_raw - P.y_med) * iv;
+                int loc = -1;
+                if (valid) {
+                    const int c = k < ncyc ? k : k - ncyc;
+                    const int r = fold_bin_in(tp, period, (double)c, bd,
+                                              inv_bd, nb);
+                    int dst;
+                    if (k < ncyc) dst = (r < nb - os || r == nb) ? r : -1;
+                    else dst = (r >= 1 && r <= os) ? r + nb - os - 1 : -1;
+                    if (dst >= s0 && dst <= hi) loc = dst - s0;
+                }
+                if (cb + 64 + lane < total) {
+                    locate(cb + 64 + lane, nk, nj);
+                    t_raw = b.t[off + nj];
+                    y_raw = load_val(b.y, b.y_dt, off + nj);
+                    if (b.dy) d_raw = load_val(b.dy, b.dy_dt, off + nj);
+                }
+                const int cnt = min(64, total - cb);
+                for (int jj = 0; jj < cnt; ++jj) {
+                    const int tj = __builtin_amdgcn_readlane(loc, jj);
+                    if (tj < 0) continue;
+                    const double yvj = readlane_d(yv, jj);
+                    const double ivj = readlane_d(iv, jj);
+                    if (tj != cur) {
+                        if (cur >= 0 && lane == 0) {
+                            my[cur] = ay;
+                            mi[cur] = ai;
+                        }
+                        cur = tj;
+                        ay = my[tj];
+                        ai = mi[tj];
+                    }
+                    ay += yvj;
+                    ai += ivj;
+                }
+            }
+        } else {
+        __syncthreads();
         // raw inputs of the first chunk; each later chunk is loaded while
         // the wave walks the current one
         double t_raw = 0.0, y_raw = 0.0, d_raw = 1.0;
@@ -1150,10 +1452,6 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
             for (int j = 0; j < cnt; ++j) {
                 const double yvj = readlane_d(yv, j), ivj = readlane_d(iv, j);
                 const int indj = __builtin_amdgcn_readlane(ind, j);
-                if (s0 == 0) {
-                    sum_y += yvj;
-                    sum_iv += ivj;
-                }
                 if (fits) {
                     // the bin's running sum stays in registers while
                     // consecutive points share it: same order of adds
@@ -1190,13 +1488,14 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
                 }
             }
         }
+        }
         __syncthreads();
         if (lane == 0) {
+            if ((fits || ranged) && cur >= 0) {
+                my[cur] = ay;
+                mi[cur] = ai;
+            }
             if (fits) {
-                if (cur >= 0) {
-                    my[cur] = ay;
-                    mi[cur] = ai;
-                }
                 for (int j = 1, d = nb - os; j <= os; ++j, ++d) {
                     my[d] = my[j];
                     mi[d] = mi[j];
@@ -1215,10 +1514,6 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
             if (!fits && s0 + S < nb) {
                 cprev_y = my[S - 1];
                 cprev_i = mi[S - 1];
-            }
-            if (s0 == 0) {
-                s_tot[0] = sum_y;
-                s_tot[1] = sum_iv;
             }
         }
         __syncthreads();
@@ -1390,7 +1685,7 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct Layout {
     size_t prm, flags, items, queues, counter, cand_lc, cand_p, t, w, y,
-        total;
+        nt, nw, ny, total;
 };
 
 Layout layout_of(const atg_bls_batch *b) {
@@ -1408,6 +1703,10 @@ Layout layout_of(const atg_bls_batch *b) {
     L.t = o;       o = align256(o + slots * sizeof(double));
     L.w = o;       o = align256(o + slots * sizeof(long long));
     L.y = o;       o = align256(o + slots * sizeof(long long));
+    const size_t npt = (size_t)b->n_points;
+    L.nt = o;      o = align256(o + npt * sizeof(double));
+    L.nw = o;      o = align256(o + npt * sizeof(long long));
+    L.ny = o;      o = align256(o + npt * sizeof(long long));
     L.total = o;
     return L;
 }
@@ -1591,6 +1890,9 @@ int atg_bls_run(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o) {
     w.ws_t = (double *)(base + L.t);
     w.ws_w = (long long *)(base + L.w);
     w.ws_y = (long long *)(base + L.y);
+    w.nat_t = (double *)(base + L.nt);
+    w.nat_w = (long long *)(base + L.nw);
+    w.nat_y = (long long *)(base + L.ny);
     w.cand_count = w.counter + 1;
     w.cand_next = w.counter + 2;
     w.cand_lc = (int *)(base + L.cand_lc);
@@ -1620,7 +1922,8 @@ int atg_bls_run(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o) {
         if (e != hipSuccess) return hip_fail(c, e, "hipFuncSetAttribute");
         c->lds_attr_set = lds;
     }
-    hipLaunchKernelGGL(k_prep, dim3(nlc), dim3(NT), 0, c->stream, kb, w);
+    hipLaunchKernelGGL(k_prep, dim3(nlc), dim3(NT), 0, c->stream, kb, w,
+                       c->tune.lds_bins);
     hipLaunchKernelGGL(k_items, dim3(1), dim3(NT), 0, c->stream,
                        (long long)b->n_lc, kb.per_off,
                        c->tune.periods_per_item, w);

@@ -174,6 +174,51 @@ def config3(n_lc=16384, seed=3, n_lo=1000, n_hi=20000, p_lo=1000,
     return out
 
 
+DURATIONS_6 = np.array([0.04, 0.06, 0.09, 0.135, 0.2, 0.3])
+
+
+def config4_curve(rng, n_points=200000, baseline=360.0, n_gaps=13,
+                  sigma=1e-3):
+    """One stitched long-baseline curve: ``n_points`` slots of the 2-min grid
+    over ``[0, baseline)`` with ``n_gaps`` periodic 1 d gaps centred at
+    ``baseline*(j+1)/(n_gaps+1)`` (a 30-min grid over 360 d has only 17,280
+    slots, so the point count is kept and the cadence is the 2-min one);
+    flux 1 + N(0, sigma) with one box at period log-uniform 1-100 d, depth
+    log-uniform 1e-4-5e-3, duration uniform 0.04-0.3 d; FP32 y/dy."""
+    step = baseline / (n_gaps + 1)
+    gaps = [(step * (j + 1) - 0.5, step * (j + 1) + 0.5)
+            for j in range(n_gaps)]
+    t = sampled_grid(rng, n_points, baseline, gaps)
+    y = 1.0 + sigma * rng.standard_normal(t.size)
+    period = float(np.exp(rng.uniform(np.log(1.0), np.log(100.0))))
+    depth = float(np.exp(rng.uniform(np.log(1e-4), np.log(5e-3))))
+    duration = float(rng.uniform(0.04, 0.3))
+    t0 = float(rng.uniform(0.0, period))
+    y = inject_box(t, y, period, t0, duration, depth)
+    truth = dict(period=period, t0=t0, duration=duration, depth=depth)
+    return (t, y.astype(np.float32), np.full(t.size, sigma, dtype=np.float32),
+            truth)
+
+
+def config4(n_lc=512, seed=4, n_points=200000, n_periods=50000):
+    """Precision-study batch: ``n_lc`` stitched 360 d curves (see
+    :func:`config4_curve`), 50,000 log-uniform periods 1-100 d, six
+    durations 0.04-0.3 d.  Bins per period reach ceil(100/0.004)+10 =
+    25,010, above the LDS histogram, so the long periods take the
+    phase-tiled path."""
+    rng = np.random.default_rng(seed)
+    ts, ys, dys, truths = [], [], [], []
+    for _ in range(n_lc):
+        t, y, dy, truth = config4_curve(rng, n_points)
+        ts.append(t)
+        ys.append(y)
+        dys.append(dy)
+        truths.append(truth)
+    return dict(t=ts, y=ys, dy=dys, truths=truths,
+                periods=np.geomspace(1.0, 100.0, n_periods),
+                durations=DURATIONS_6.copy())
+
+
 def config5_pool(n=128, seed=5, n_lo=1000, n_hi=200000, base_lo=10.0,
                  base_hi=360.0, p_cap=20000):
     """Pool of serving requests: N log-uniform 1k-200k, baseline uniform

@@ -146,6 +146,114 @@ def serve5_leg(seconds, objective, rate=2000.0, pool=96, budget=16e9):
             "statuses_ok": int(sum(h.status == 0 for h in handles))}
 
 
+def _err_stats(got, ref):
+    """Errors of one curve's per-period outputs against the FP64 oracle."""
+    fin = np.isfinite(ref["power"]) & (ref["power"] != 0)
+    g = got["power"][fin]
+    r = ref["power"][fin]
+    rel = np.abs(g - r) / np.abs(r)
+    ib = int(np.argmax(np.where(np.isfinite(ref["power"]), ref["power"],
+                                -np.inf)))
+    gb = int(np.argmax(np.where(np.isfinite(got["power"]), got["power"],
+                                -np.inf)))
+    same = (got["duration"] == ref["duration"]) & (
+        np.abs(got["transit_time"] - ref["transit_time"]) <= 1e-6)
+    return dict(rel=rel, best_same=gb == ib,
+                dt0_best=abs(got["transit_time"][ib] - ref["transit_time"][ib]),
+                ddepth_best=abs(got["depth"][ib] - ref["depth"][ib]) /
+                max(abs(ref["depth"][ib]), 1e-300),
+                window_same=float(np.mean(same[np.isfinite(ref["power"])])))
+
+
+def _summarise(stats):
+    rel = np.concatenate([s["rel"] for s in stats])
+    return {"power_rel_max": float("%.3g" % rel.max()),
+            "power_rel_p50": float("%.3g" % np.median(rel)),
+            "power_rel_p99": float("%.3g" % np.percentile(rel, 99)),
+            "frac_periods_rel_gt_1e-6": float("%.4g" % np.mean(rel > 1e-6)),
+            "best_period_same": "%d/%d" % (sum(s["best_same"] for s in stats),
+                                           len(stats)),
+            "t0_err_at_best_max_d": float("%.3g" % max(s["dt0_best"]
+                                                       for s in stats)),
+            "depth_rel_err_at_best_max": float("%.3g" % max(
+                s["ddepth_best"] for s in stats)),
+            "window_agreement_min": float("%.4g" % min(s["window_same"]
+                                                       for s in stats))}
+
+
+def precision_leg(n_lc, n_sample, objective, seconds_cap=None):
+    """Precision study on the long-baseline workload: 512 stitched curves x
+    200,000 points over 360 d, 50,000 log-uniform periods 1-100 d, six
+    durations.  The GPU runs the whole batch (FP64 fold, int64 bins) in
+    parity and fast mode; on ``n_sample`` curves every output is compared
+    with the FP64 oracle, beside the single-precision alternatives (FP32
+    fold, FP32 bin sums and box scan) evaluated by the oracle's variants on
+    the host."""
+    import torch
+    from oracle import oracle
+    from astro_transit_gpu_amd import Context, DeviceBatch, synth
+    t0 = time.perf_counter()
+    d = synth.config4(n_lc=n_lc, seed=4)
+    gen = time.perf_counter() - t0
+    print("precision: generated %d curves in %.1f s" % (n_lc, gen),
+          file=sys.stderr, flush=True)
+    ctx = Context(0)
+    b = DeviceBatch.from_host(d["t"], d["y"], d["dy"], d["periods"],
+                              d["durations"], device=ctx.device)
+    out = {}
+    gpu = {}
+    for mode in ("parity", "fast"):
+        torch.cuda.synchronize()
+        ctx.timing(True, True)
+        s = time.perf_counter()
+        res = ctx.run(b, objective=objective, mode=mode)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - s
+        _, kms = ctx.timing(False, True)
+        h = res.to_host()
+        out[mode] = [{k: v.copy() for k, v in res.curve(i, h).items()}
+                     for i in range(n_sample)]
+        gpu[mode] = {"LC_per_s": round(n_lc / el, 2), "seconds": round(el, 2),
+                     "k_main_ms": round(kms, 1),
+                     "statuses_ok": int((h["status"] == 0).sum())}
+        print("precision: GPU %s %.1f s" % (mode, el), file=sys.stderr,
+              flush=True)
+        del res, h
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    stats = {k: [] for k in ("gpu_parity_f64fold_i64bins",
+                             "gpu_fast_f64fmafold_i64bins",
+                             "fp32_fold", "fp32_bins_and_scan",
+                             "fp32_fold_bins_scan")}
+    cpu_s = 0.0
+    for i in range(n_sample):
+        s = time.perf_counter()
+        ref = oracle.power(d["t"][i], d["y"][i], d["dy"][i], d["periods"],
+                           d["durations"], objective, n_threads=cores)
+        cpu_s += time.perf_counter() - s
+        stats["gpu_parity_f64fold_i64bins"].append(
+            _err_stats(out["parity"][i], ref))
+        stats["gpu_fast_f64fmafold_i64bins"].append(
+            _err_stats(out["fast"][i], ref))
+        for name, f32, a32 in (("fp32_fold", 1, 0),
+                               ("fp32_bins_and_scan", 0, 1),
+                               ("fp32_fold_bins_scan", 1, 1)):
+            v = oracle.power_variant(d["t"][i], d["y"][i], d["dy"][i],
+                                     d["periods"], d["durations"], objective,
+                                     fold32=f32, acc32=a32, n_threads=cores)
+            stats[name].append(_err_stats(v, ref))
+        print("precision: curve %d compared" % i, file=sys.stderr,
+              flush=True)
+    return {"metric": "precision study: error of each result vs the FP64 "
+            "oracle", "workload": "%d curves x 200,000 pts over 360 d "
+            "(2-min grid, 13 periodic 1 d gaps, FP32 y/dy) x 50,000 "
+            "log-uniform periods 1-100 d x 6 durations 0.04-0.3 d, %s" % (
+                n_lc, objective),
+            "compared_curves": n_sample, "gpu": gpu,
+            "errors": {k: _summarise(v) for k, v in stats.items()},
+            "cpu_oracle_LC_per_s": round(n_sample / cpu_s, 4),
+            "cpu_cores": cores, "generation_s": round(gen, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +270,10 @@ def main():
     ap.add_argument("--skip-extras", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tiling")
+    ap.add_argument("--precision", action="store_true",
+                    help="run only the long-baseline precision study")
+    ap.add_argument("--precision-lc", type=int, default=512)
+    ap.add_argument("--precision-sample", type=int, default=4)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy first (a kernel "
                     "of known HBM traffic for counter calibration)")
@@ -171,6 +283,12 @@ def main():
               "path (curves and periods shard independently; see the design "
               "notes).", file=sys.stderr)
         sys.exit(2)
+
+    if args.precision:
+        print(json.dumps(precision_leg(args.precision_lc,
+                                       args.precision_sample, "likelihood")),
+              flush=True)
+        return
 
     import torch
     from astro_transit_gpu_amd import (BoxLeastSquaresGPU, Context,

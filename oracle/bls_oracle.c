@@ -193,3 +193,171 @@ int atg_oracle_bls(int64_t n, const double *t, const double *y,
 }
 
 int atg_oracle_abi_version(void) { return 1; }
+
+
+/*
+ * Precision-study variants (what the GPU design avoids; no astropy
+ * counterpart).  They restate the reference's single-precision V42 kernel
+ * (SCALAR_T = float, src/astrotransit_gpu/search/kernels/vbls_v42_parity.cu:
+ * 36-46 grid constants, 68-74 fold and binning, 78-84 padding and prefix
+ * sums, 105-125 box scan; totals summed in float, v42_parity.py:66-67):
+ *   fold32: t' = t - t_ref, the period, 1/P, bd and 1/bd in float and
+ *           ph = t' - P*floor(t'*(1/P)), bin = (int)(ph*(1/bd)) + 1 in float;
+ *   acc32:  the per-point products, the bins, the padding, the cumulative
+ *           sums, the totals and the box scan in float.
+ * With both flags 0 the result equals atg_oracle_bls (checked by the tests).
+ */
+static void one_period_variant(int64_t n, const double *t, const double *y,
+                               const double *ivar, double sum_y,
+                               double sum_ivar, float sy32, float si32,
+                               double period, double bin_duration,
+                               int64_t n_durations, const double *durations,
+                               int oversample, int use_likelihood,
+                               int fold32, int acc32, double *mean_y,
+                               double *mean_ivar, float *fy, float *fi,
+                               double *o_power, double *o_depth,
+                               double *o_duration, double *o_phase)
+{
+    int n_bins = (int)ceil(period / bin_duration) + oversample;
+    const float pf = (float)period, ipf = 1.0f / pf;
+    const float bdf = (float)bin_duration, ibdf = 1.0f / bdf;
+    if (fold32) n_bins = (int)ceilf(pf * ibdf) + oversample;
+    for (int b = 0; b <= n_bins; ++b) {
+        mean_y[b] = mean_ivar[b] = 0.0;
+        fy[b] = fi[b] = 0.0f;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        int ind;
+        if (fold32) {
+            const float tf = (float)t[i];
+            const float ph = tf - pf * floorf(tf * ipf);
+            ind = (int)(ph * ibdf) + 1;
+        } else {
+            ind = (int)(wrap_into(t[i], period) / bin_duration) + 1;
+        }
+        if (ind < 0) ind = 0;
+        if (ind > n_bins) ind = n_bins;
+        if (acc32) {
+            const float wf = (float)ivar[i];
+            fi[ind] += wf;
+            fy[ind] += (float)y[i] * wf;
+        } else {
+            mean_y[ind] += y[i] * ivar[i];
+            mean_ivar[ind] += ivar[i];
+        }
+    }
+    for (int j = 1, ind = n_bins - oversample; j <= oversample; ++j, ++ind) {
+        mean_y[ind] = mean_y[j];
+        mean_ivar[ind] = mean_ivar[j];
+        fy[ind] = fy[j];
+        fi[ind] = fi[j];
+    }
+    for (int b = 1; b <= n_bins; ++b) {
+        mean_y[b] += mean_y[b - 1];
+        mean_ivar[b] += mean_ivar[b - 1];
+        fy[b] += fy[b - 1];
+        fi[b] += fi[b - 1];
+    }
+    const double bdv = fold32 ? (double)bdf : bin_duration;
+    double best = -INFINITY, b_depth = 0.0, b_dur = 0.0, b_phase = 0.0;
+    for (int64_t k = 0; k < n_durations; ++k) {
+        const int dur = fold32 ? (int)roundf((float)durations[k] * ibdf)
+                               : (int)round(durations[k] / bin_duration);
+        for (int s = 0; s <= n_bins - dur; ++s) {
+            double depth, obj;
+            if (acc32) {
+                float y_in = fy[s + dur] - fy[s];
+                float ivar_in = fi[s + dur] - fi[s];
+                float y_out = sy32 - y_in;
+                float ivar_out = si32 - ivar_in;
+                if (ivar_in < (float)DBL_EPSILON ||
+                    ivar_out < (float)DBL_EPSILON)
+                    continue;
+                y_in /= ivar_in;
+                y_out /= ivar_out;
+                const float d = y_out - y_in;
+                const float snr = d / sqrtf(1.0f / ivar_in + 1.0f / ivar_out);
+                depth = d;
+                obj = use_likelihood ? 0.5 * (double)(ivar_in * d * d)
+                                     : (double)snr;
+            } else {
+                double y_in = mean_y[s + dur] - mean_y[s];
+                double ivar_in = mean_ivar[s + dur] - mean_ivar[s];
+                double y_out = sum_y - y_in;
+                double ivar_out = sum_ivar - ivar_in;
+                if (ivar_in < DBL_EPSILON || ivar_out < DBL_EPSILON) continue;
+                y_in /= ivar_in;
+                y_out /= ivar_out;
+                depth = y_out - y_in;
+                obj = use_likelihood
+                          ? 0.5 * ivar_in * depth * depth
+                          : depth / sqrt(1.0 / ivar_in + 1.0 / ivar_out);
+            }
+            if (depth >= 0.0 && obj > best) {
+                best = obj;
+                b_depth = depth;
+                b_dur = dur * bdv;
+                b_phase = fmod(s * bdv + 0.5 * b_dur, period);
+            }
+        }
+    }
+    *o_power = best;
+    *o_depth = b_depth;
+    *o_duration = b_dur;
+    *o_phase = b_phase;
+}
+
+int atg_oracle_bls_variant(int64_t n, const double *t, const double *y,
+                           const double *ivar, int64_t n_periods,
+                           const double *periods, int64_t n_durations,
+                           const double *durations, int oversample,
+                           int use_likelihood, int fold32, int acc32,
+                           int n_threads, double *power, double *depth,
+                           double *duration, double *transit_phase)
+{
+    if (n_periods <= 0 || n_durations <= 0 || oversample < 1) return 3;
+    double min_dur = durations[0], max_period = periods[0];
+    for (int64_t k = 1; k < n_durations; ++k)
+        if (durations[k] < min_dur) min_dur = durations[k];
+    for (int64_t p = 1; p < n_periods; ++p)
+        if (periods[p] > max_period) max_period = periods[p];
+    double bin_duration = min_dur / (double)oversample;
+    if (fold32) bin_duration = (double)((float)min_dur / (float)oversample);
+    /* +2: the float bin count may exceed the double one by one */
+    int max_n_bins = (int)ceil(max_period / bin_duration) + oversample + 2;
+    double sum_y = 0.0, sum_ivar = 0.0;
+    float sy32 = 0.0f, si32 = 0.0f;
+    for (int64_t i = 0; i < n; ++i) {
+        sum_y += y[i] * ivar[i];
+        sum_ivar += ivar[i];
+        sy32 += (float)y[i] * (float)ivar[i];
+        si32 += (float)ivar[i];
+    }
+#ifdef _OPENMP
+    if (n_threads <= 0) n_threads = omp_get_max_threads();
+#else
+    n_threads = 1;
+#endif
+#pragma omp parallel num_threads(n_threads)
+    {
+        size_t nb1 = (size_t)(max_n_bins + 1);
+        double *my = (double *)malloc(sizeof(double) * nb1);
+        double *mi = (double *)malloc(sizeof(double) * nb1);
+        float *fy = (float *)malloc(sizeof(float) * nb1);
+        float *fi = (float *)malloc(sizeof(float) * nb1);
+#pragma omp for schedule(dynamic, 8)
+        for (int64_t p = 0; p < n_periods; ++p) {
+            if (!my || !mi || !fy || !fi) continue;
+            one_period_variant(n, t, y, ivar, sum_y, sum_ivar, sy32, si32,
+                               periods[p], bin_duration, n_durations,
+                               durations, oversample, use_likelihood, fold32,
+                               acc32, my, mi, fy, fi, power + p, depth + p,
+                               duration + p, transit_phase + p);
+        }
+        free(my);
+        free(mi);
+        free(fy);
+        free(fi);
+    }
+    return 0;
+}

@@ -85,6 +85,37 @@ def bls_impl(t_rel, y_centered, ivar, periods, durations, oversample=10,
     return dict(zip(FIELDS, outs))
 
 
+def power_variant(t, y, dy, periods, durations, objective="likelihood",
+                  oversample=10, fold32=False, acc32=False, n_threads=0):
+    """Precision-study variant of :func:`power`: FP32 phase fold and/or FP32
+    bin accumulation (see atg_oracle_bls_variant).  Returns power, depth,
+    duration and transit_time."""
+    lib = _load()
+    if not hasattr(lib, "_variant_ready"):
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.atg_oracle_bls_variant.restype = ctypes.c_int
+        lib.atg_oracle_bls_variant.argtypes = [
+            ctypes.c_int64, dp, dp, dp, ctypes.c_int64, dp, ctypes.c_int64,
+            dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+            ctypes.c_int] + [dp] * 4
+        lib._variant_ready = True
+    t_rel, yc, ivar, t_ref = prepare(t, y, dy)
+    periods = np.ascontiguousarray(np.atleast_1d(periods), dtype=np.float64)
+    durations = np.ascontiguousarray(np.atleast_1d(durations),
+                                     dtype=np.float64)
+    outs = [np.empty(periods.size) for _ in range(4)]
+    rc = lib.atg_oracle_bls_variant(
+        t_rel.size, _p(t_rel), _p(yc), _p(ivar), periods.size, _p(periods),
+        durations.size, _p(durations), int(oversample),
+        int(objective == "likelihood"), int(bool(fold32)), int(bool(acc32)),
+        int(n_threads), *[_p(o) for o in outs])
+    if rc != 0:
+        raise RuntimeError("oracle variant failed with code %d" % rc)
+    out = dict(zip(("power", "depth", "duration", "transit_time"), outs))
+    out["transit_time"] = out["transit_time"] + t_ref
+    return out
+
+
 def prepare(t, y, dy):
     """astropy power() input preparation (core.py:304-311, 327):
     FP64 t, t_ref = min(t), ivar = 1/dy^2 (ones if dy is None),

@@ -135,11 +135,12 @@ def test_status_codes(ctx):
 
 
 @pytest.mark.parametrize("name", ["cfg1_small_likelihood", "cfg1_small_snr",
-                                  "hetero_dy"])
+                                  "hetero_dy", "unsorted_t"])
 def test_segmented_bins_bit_identical(name):
     """Periods that do not fit the LDS histogram go through the
     phase-tiled path; with a tiny LDS budget every period does, and the
-    outputs must equal the in-LDS path bit for bit."""
+    outputs must equal the in-LDS path bit for bit.  Time-sorted curves
+    take the ranged tiles, unsorted_t the refolding ones."""
     from astro_transit_gpu_amd import Context
     c = O.load_golden([p for p in GOLDEN if p.endswith(name + ".npz")][0])
     big = Context(0)
@@ -193,3 +194,34 @@ def test_hipgraph_replays_match_eager(ctx):
             assert np.array_equal(h[k], eager[k]), (rep, k)
         ctx.run(b, objective="snr")  # unrelated eager work between replays
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("mode", ["parity", "fast"])
+def test_long_baseline_parity(ctx, mode):
+    """Two 200,000-point curves over 360 d with 13 gaps (the precision-study
+    workload) on every 100th of its 50,000 periods 1-100 d: the long periods
+    need up to 25,010 bins and take the phase-tiled path.  Parity mode holds
+    the parity bar with every best-period field equal to the oracle's; fast
+    mode holds the same power and T0 bar."""
+    from astro_transit_gpu_amd import DeviceBatch, synth
+    rng = np.random.default_rng(404)
+    curves = [synth.config4_curve(rng) for _ in range(2)]
+    periods = np.geomspace(1.0, 100.0, 50000)[::100]
+    durs = synth.DURATIONS_6
+    print("long baseline", mode, flush=True)
+    b = DeviceBatch.from_host([c[0] for c in curves], [c[1] for c in curves],
+                              [c[2] for c in curves], periods, durs,
+                              device=ctx.device)
+    res = ctx.run(b, objective="likelihood", mode=mode)
+    h = res.to_host()
+    assert list(h["status"]) == [0, 0]
+    for i, c in enumerate(curves):
+        ref = O.power(c[0], c[1], c[2], periods, durs, "likelihood")
+        g = res.curve(i, h)
+        rel = assert_parity(g, ref, "curve %d" % i)
+        print("%s curve %d rel %.2e agree %.4f" % (
+            mode, i, rel, window_agreement(g, ref)))
+        if mode == "parity":
+            ib = int(np.argmax(ref["power"]))
+            for k in ref:
+                assert g[k][ib] == ref[k][ib], (i, k)
