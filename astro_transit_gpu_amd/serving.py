@@ -119,9 +119,9 @@ class BLSServer:
         ytd = torch.float32 if self.y_dtype == np.float32 else torch.float64
         ysz = self.y_dtype.itemsize
         if hbm_budget_bytes is not None:
-            # bytes per typical request: inputs + engine workspace + grid
-            # + 7 FP64 outputs
-            per_req = typical_n * (8 + 2 * ysz + 24) + typical_p * (8 + 56)
+            # bytes per typical request: inputs + engine workspace (two
+            # 24 B/point layouts) + grid + 7 FP64 outputs
+            per_req = typical_n * (8 + 2 * ysz + 48) + typical_p * (8 + 56)
             n_req = max(1, int(hbm_budget_bytes // per_req))
             capacity_points = capacity_points or n_req * typical_n
             capacity_periods = capacity_periods or n_req * typical_p
