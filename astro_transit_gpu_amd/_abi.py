@@ -31,7 +31,7 @@ EXPORTS = (
     "atg_abi_version", "atg_ctx_create", "atg_ctx_destroy",
     "atg_ctx_set_stream", "atg_last_error", "atg_ctx_set_tuning",
     "atg_ctx_get_tuning", "atg_bls_workspace_size", "atg_bls_run",
-    "atg_bls_run_sync", "atg_timing_main_kernel",
+    "atg_bls_run_sync", "atg_timing_main_kernel", "atg_bls_top_k",
 )
 
 _vp = ctypes.c_void_p
@@ -99,6 +99,10 @@ def lib():
     for fn in (L.atg_bls_run, L.atg_bls_run_sync):
         fn.restype = ctypes.c_int
         fn.argtypes = [_vp, ctypes.POINTER(Batch), ctypes.POINTER(Out)]
+    L.atg_bls_top_k.restype = ctypes.c_int
+    L.atg_bls_top_k.argtypes = [_vp, ctypes.POINTER(Batch),
+                                ctypes.POINTER(Out), ctypes.c_int,
+                                ctypes.c_int, _vp, _vp]
     L.atg_timing_main_kernel.restype = ctypes.c_int
     L.atg_timing_main_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_double)]

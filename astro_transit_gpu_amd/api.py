@@ -10,11 +10,16 @@ Mirrors src/astrotransit_gpu/search/api.py:
     same validation messages (api.py:51-70).
 
 Both methods compute astropy's BoxLeastSquares objective (the reference's
-oracle, search/cpu_reference_bls.py:11-16) on the GPU:
-  * ``method="parity"``  int64 fixed-point bins, bit-reproducible run to
-    run, within 1e-6 (relative power) of the FP64 oracle;
-  * ``method="fast"``    the same search with FP32 bin accumulation (LDS
-    float atomics): a stated drift from the oracle, not bit-reproducible.
+oracle, search/cpu_reference_bls.py:11-16) on the GPU with an FP64 phase
+fold and int64 fixed-point bins, so both are bit-reproducible run to run:
+  * ``method="parity"``  the oracle's exact bin rule and an oracle-order
+    recomputation of the near-best periods: within 1e-6 (relative power)
+    of the FP64 oracle, the best period's fields equal to it;
+  * ``method="fast"``    a fused-multiply-add fold without the bin-edge
+    test and no recomputation: a point exactly on a bin edge may land in
+    the neighbouring bin (a stated drift).
+The top candidates come from the device top-k (``Context.top_k``), with
+the rule of the reference's get_top_k_candidates.
 """
 from dataclasses import dataclass, field
 from typing import Any, Dict, List, Optional
@@ -112,7 +117,7 @@ class BoxLeastSquaresGPU:
             n_bins: accepted for compatibility; the phase grid follows the
                 oracle's rule (bins of min(durations)/oversample).
             dtype: accepted for compatibility; the fold is FP64 and the bins
-                int64 (parity) or FP32 (fast) whatever the dtype.
+                int64 whatever the dtype.
             method: 'parity' (bit-reproducible) or 'fast'.
             objective: 'snr' (the reference's oracle, cpu_reference_bls.py:
                 16) or 'likelihood' (astropy's default).
@@ -156,19 +161,20 @@ class BoxLeastSquaresGPU:
         batch = DeviceBatch.from_host(
             [t], [y], None if dy is None else [dy], periods, durations,
             device=ctx.device, y_dtype=np.float32 if y32 else np.float64)
-        res = ctx.run(batch, objective=objective, mode=mode,
-                      oversample=oversample).to_host()
+        dres = ctx.run(batch, objective=objective, mode=mode,
+                       oversample=oversample, sync=False)
+        tk_idx, tk_cnt = ctx.top_k(batch, dres, k=5, distance=10, sync=False)
+        res = dres.to_host()
         status = int(res["status"][0])
         if status != 0:
             from ._abi import LC_STATUS
             raise ValueError("BLS failed: %s" % LC_STATUS.get(status, status))
         best = int(res["best_index"][0])
         f = {k: res[k] for k in FIELDS}
-        raw = dict(power=f["power"], periods=periods,
-                   all_t0s=f["transit_time"], all_durs=f["duration"],
-                   all_depths=f["depth"])
-        top = [Candidate(c["period"], c["t0"], c["duration"], c["depth"],
-                         c["power"]) for c in get_top_k_candidates(raw, k=5)]
+        top = [Candidate(float(periods[i]), float(f["transit_time"][i]),
+                         float(f["duration"][i]), float(f["depth"][i]),
+                         float(f["power"][i]))
+               for i in tk_idx[0, :int(tk_cnt[0])].tolist()]
         return BLSResult(
             best_period=float(periods[best]),
             best_t0=float(f["transit_time"][best]),

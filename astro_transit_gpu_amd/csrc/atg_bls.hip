@@ -1583,6 +1583,185 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
     }  // queue loop
 }
 
+// ------------------------------------------------------------------ k_topk
+//
+// Peaks of one curve's periodogram with the rule of scipy's find_peaks
+// (local maxima; a plateau gives its midpoint; first and last samples never
+// count) thinned by the minimum distance, then the k highest kept peaks.
+// scipy thins greedily from the highest peak down; here the same result
+// comes from rounds: an undecided peak is kept when no kept peak lies within
+// the distance and it ranks above every undecided peak there, and an
+// undecided peak within the distance of a kept one is dropped.  A peak is
+// only ever dropped by a kept peak of higher rank, and the highest
+// undecided peak is decided in every round, so the rounds end with exactly
+// the greedy selection.  Rank: power, then period index (higher first).
+
+struct Cand {
+    double v;
+    int i;
+};
+
+// higher rank: larger value, then larger index; i < 0 ranks lowest
+__device__ __forceinline__ bool cand_above(double v, int i, double bv,
+                                           int bi) {
+    if (i < 0) return false;
+    if (bi < 0) return true;
+    return v > bv || (v == bv && i > bi);
+}
+
+__device__ Cand block_best(Cand c, double *dscr, int *iscr) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(c.v, o);
+        const int oi = __shfl_xor(c.i, o);
+        if (cand_above(ov, oi, c.v, c.i)) {
+            c.v = ov;
+            c.i = oi;
+        }
+    }
+    const int wv = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        dscr[wv] = c.v;
+        iscr[wv] = c.i;
+    }
+    __syncthreads();
+    Cand r{dscr[0], iscr[0]};
+    for (int k = 1; k < NW; ++k)
+        if (cand_above(dscr[k], iscr[k], r.v, r.i)) {
+            r.v = dscr[k];
+            r.i = iscr[k];
+        }
+    return r;
+}
+
+__global__ __launch_bounds__(NT) void k_topk(const long long *per_off,
+                                             const double *power,
+                                             const int *status, int *tk_pos,
+                                             unsigned char *tk_st, int k,
+                                             int dist, long long *idx_out,
+                                             int *cnt_out) {
+    __shared__ int iscr[NW];
+    __shared__ double dscr[NW];
+    __shared__ int s_pick[64];
+    const int lc = blockIdx.x, tid = threadIdx.x;
+    const long long po = per_off[lc];
+    const int np = (int)(per_off[lc + 1] - po);
+    const double *x = power + po;
+    int *pos = tk_pos + po;
+    unsigned char *st = tk_st + po;
+    if (tid < k) idx_out[(long long)lc * k + tid] = -1;
+    if (status && status[lc] != ATG_LC_OK) {
+        if (tid == 0) cnt_out[lc] = 0;
+        return;
+    }
+    // local maxima in order of their left edges (= order of midpoints)
+    int npk = 0;
+    for (int base = 1; base < np - 1; base += NT) {
+        const int i = base + tid;
+        int mid = -1;
+        if (i < np - 1 && x[i - 1] < x[i]) {
+            int a = i + 1;
+            while (a < np - 1 && x[a] == x[i]) ++a;
+            if (x[a] < x[i]) mid = (i + a - 1) / 2;
+        }
+        int tot;
+        const int e = block_excl_scan(mid >= 0 ? 1 : 0, iscr, &tot);
+        if (mid >= 0) {
+            pos[npk + e] = mid;
+            st[npk + e] = 0;
+        }
+        npk += tot;
+    }
+    __syncthreads();
+    int count = 0;
+    if (npk > 0) {
+        // 0 undecided, 1 kept, 2 dropped, 3 already reported
+        for (;;) {
+            for (int j = tid; j < npk; j += NT) {
+                if (st[j] != 0) continue;
+                const double vj = x[pos[j]];
+                bool ok = true;
+                for (int q = j - 1; ok && q >= 0 && pos[j] - pos[q] < dist;
+                     --q)
+                    ok = !(st[q] == 1 ||
+                           (st[q] == 0 && cand_above(x[pos[q]], pos[q], vj,
+                                                     pos[j])));
+                for (int q = j + 1; ok && q < npk && pos[q] - pos[j] < dist;
+                     ++q)
+                    ok = !(st[q] == 1 ||
+                           (st[q] == 0 && cand_above(x[pos[q]], pos[q], vj,
+                                                     pos[j])));
+                if (ok) st[j] = 1;
+            }
+            __syncthreads();
+            int und = 0;
+            for (int j = tid; j < npk; j += NT) {
+                if (st[j] != 0) continue;
+                bool drop = false;
+                for (int q = j - 1; !drop && q >= 0 && pos[j] - pos[q] < dist;
+                     --q)
+                    drop = st[q] == 1;
+                for (int q = j + 1; !drop && q < npk && pos[q] - pos[j] < dist;
+                     ++q)
+                    drop = st[q] == 1;
+                if (drop) st[j] = 2;
+                else ++und;
+            }
+            und = block_reduce(und, AddOp<int>(), iscr);
+            if (und == 0) break;
+        }
+        // the k kept peaks of highest rank
+        for (int r = 0; r < k; ++r) {
+            Cand c{-INFINITY, -1};
+            for (int j = tid; j < npk; j += NT)
+                if (st[j] == 1 && cand_above(x[pos[j]], pos[j], c.v, c.i)) {
+                    c.v = x[pos[j]];
+                    c.i = pos[j];
+                }
+            const Cand bst = block_best(c, dscr, iscr);
+            if (bst.i < 0) break;
+            for (int j = tid; j < npk; j += NT)
+                if (pos[j] == bst.i) st[j] = 3;
+            if (tid == 0) idx_out[(long long)lc * k + r] = bst.i;
+            ++count;
+            __syncthreads();
+        }
+    } else {
+        // no peak: the k highest values, listed in decreasing value with the
+        // earlier index first among equal values
+        for (int r = 0; r < k && r < np; ++r) {
+            Cand c{-INFINITY, -1};
+            for (int i = tid; i < np; i += NT) {
+                bool taken = false;
+                for (int q = 0; q < r; ++q) taken |= s_pick[q] == i;
+                if (!taken && cand_above(x[i], i, c.v, c.i)) {
+                    c.v = x[i];
+                    c.i = i;
+                }
+            }
+            const Cand bst = block_best(c, dscr, iscr);
+            if (tid == 0) s_pick[r] = bst.i;
+            __syncthreads();
+            ++count;
+        }
+        if (tid == 0) {
+            for (int a = 1; a < count; ++a) {  // insertion sort, k <= 64
+                const int v = s_pick[a];
+                int z = a - 1;
+                while (z >= 0 && (x[s_pick[z]] < x[v] ||
+                                  (x[s_pick[z]] == x[v] && s_pick[z] > v))) {
+                    s_pick[z + 1] = s_pick[z];
+                    --z;
+                }
+                s_pick[z + 1] = v;
+            }
+            for (int r = 0; r < count; ++r)
+                idx_out[(long long)lc * k + r] = s_pick[r];
+        }
+    }
+    if (tid == 0) cnt_out[lc] = count;
+}
+
 // ----------------------------------------------------------------- k_final
 
 __global__ __launch_bounds__(NT) void k_final(Batch b, Out o, Work w) {
@@ -1685,7 +1864,7 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct Layout {
     size_t prm, flags, items, queues, counter, cand_lc, cand_p, t, w, y,
-        nt, nw, ny, total;
+        nt, nw, ny, tk_pos, tk_st, total;
 };
 
 Layout layout_of(const atg_bls_batch *b) {
@@ -1707,6 +1886,9 @@ Layout layout_of(const atg_bls_batch *b) {
     L.nt = o;      o = align256(o + npt * sizeof(double));
     L.nw = o;      o = align256(o + npt * sizeof(long long));
     L.ny = o;      o = align256(o + npt * sizeof(long long));
+    const size_t nper = (size_t)b->n_periods;
+    L.tk_pos = o;  o = align256(o + nper * sizeof(int));
+    L.tk_st = o;   o = align256(o + nper);
     L.total = o;
     return L;
 }
@@ -1963,6 +2145,36 @@ int atg_bls_run_sync(atg_ctx *c, const atg_bls_batch *b,
     if (rc != ATG_OK) return rc;
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "hipStreamSynchronize");
+    return ATG_OK;
+}
+
+int atg_bls_top_k(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
+                  int k, int distance, int64_t *index_out,
+                  int32_t *count_out) {
+    if (!c) return ATG_ERR_ARG;
+    if (!b || !o || !o->power || !index_out || !count_out ||
+        !b->period_offsets)
+        return fail(c, ATG_ERR_ARG, "null batch, power or output pointer");
+    if (b->n_lc < 1 || b->n_lc > INT_MAX || b->n_periods < 1)
+        return fail(c, ATG_ERR_ARG, "n_lc and n_periods must be >= 1");
+    if (b->n_periods > INT_MAX)
+        return fail(c, ATG_ERR_ARG, "n_periods must be < 2^31");
+    if (k < 1 || k > 64) return fail(c, ATG_ERR_ARG, "k must be in [1, 64]");
+    if (distance < 1) return fail(c, ATG_ERR_ARG, "distance must be >= 1");
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+    const Layout L = layout_of(b);
+    int rc = ensure_workspace(c, L.total);
+    if (rc != ATG_OK) return rc;
+    char *base = (char *)c->ws;
+    hipLaunchKernelGGL(k_topk, dim3((unsigned)b->n_lc), dim3(NT), 0,
+                       c->stream, (const long long *)b->period_offsets,
+                       (const double *)o->power, (const int *)o->status,
+                       (int *)(base + L.tk_pos),
+                       (unsigned char *)(base + L.tk_st), k, distance,
+                       (long long *)index_out, (int *)count_out);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(c, e, "kernel launch");
     return ATG_OK;
 }
 

@@ -284,6 +284,43 @@ class Context:
             self.set_stream(prev)
         return g
 
+    def top_k(self, batch: DeviceBatch, result: DeviceResult, k=5,
+              distance=10, sync=True):
+        """Top-k peaks of every curve's periodogram, on the device
+        (atg_bls_top_k; the rule of the reference's get_top_k_candidates,
+        search/gpu_bls.py:146-174).  Returns (index, count): int64 (n_lc, k)
+        period indices relative to each curve's grid, -1 past count[c]; int32
+        (n_lc,) candidates per curve (0 for a curve whose status is not OK).
+        """
+        idx = torch.empty((batch.n_lc, int(k)), dtype=torch.int64,
+                          device=self.device)
+        cnt = torch.empty(batch.n_lc, dtype=torch.int32, device=self.device)
+        b, o = self._structs(batch, result, "likelihood", "parity", 10)
+        self._check(self._lib.atg_bls_top_k(
+            self._h, ctypes.byref(b), ctypes.byref(o), int(k), int(distance),
+            _ptr(idx), _ptr(cnt)))
+        if sync:
+            self.stream.synchronize()
+        return idx, cnt
+
+    def candidates(self, batch: DeviceBatch, result: DeviceResult, k=5,
+                   distance=10):
+        """Per curve, the top-k candidates as the reference's dicts
+        (period, t0, duration, depth, power; gpu_bls.py:164-173), gathered
+        on the device so that only k values per field leave it."""
+        idx, cnt = self.top_k(batch, result, k, distance, sync=False)
+        off = torch.as_tensor(batch.host_period_offsets[:-1],
+                              device=self.device).unsqueeze(1)
+        g = (idx.clamp(min=0) + off).reshape(-1)
+        cols = [batch.periods, result.transit_time, result.duration,
+                result.depth, result.power]
+        vals = torch.stack([c[g] for c in cols], 1).reshape(
+            batch.n_lc, int(k), 5).cpu().numpy()
+        cnt = cnt.cpu().numpy()
+        keys = ("period", "t0", "duration", "depth", "power")
+        return [[dict(zip(keys, map(float, vals[c, r])))
+                 for r in range(int(cnt[c]))] for c in range(batch.n_lc)]
+
     def workspace_bytes(self, batch: DeviceBatch):
         b, _ = self._structs(batch, DeviceResult.empty(
             1, 1, batch.host_period_offsets, self.device), "likelihood",

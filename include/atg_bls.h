@@ -152,11 +152,32 @@ int atg_bls_workspace_size(const atg_ctx *ctx, const atg_bls_batch *batch,
 int atg_bls_run(atg_ctx *ctx, const atg_bls_batch *batch,
                 const atg_bls_out *out);
 
-/* atg_bls_run followed by a stream synchronisation; also reports the
- * first non-OK light-curve status as ATG_ERR_LDS when a period needed more
- * bins than the LDS budget (other statuses are left to the caller). */
+/* atg_bls_run followed by a synchronisation of the context stream.
+ * Per-curve problems stay in out->status. */
 int atg_bls_run_sync(atg_ctx *ctx, const atg_bls_batch *batch,
                      const atg_bls_out *out);
+
+/* Top-k peaks of every curve's periodogram (asynchronous on the context
+ * stream).  Replaces get_top_k_candidates (search/gpu_bls.py:146-174) as
+ * BoxLeastSquaresGPU.power uses it (search/api.py:113-123): the local
+ * maxima of out->power (plateaus give their midpoint) thinned so that no
+ * two kept peaks are closer than `distance` periods, highest power first
+ * (scipy.signal.find_peaks(power, distance=distance)), then the k kept
+ * peaks of highest power, in decreasing power.  A curve without a peak
+ * gives its k highest powers instead.  Equal powers rank the later period
+ * first (the order of the reference's reversed stable sorts; in the
+ * no-peak case the final order puts the earlier period first).
+ *   index_out[c*k + r]: period index of the r-th candidate of curve c,
+ *                       relative to the curve's first period; -1 past
+ *                       count_out[c] candidates.
+ *   count_out[c]:       candidates found (0 for a curve whose status in
+ *                       out->status, if given, is not ATG_LC_OK).
+ * batch gives n_lc, n_periods and period_offsets; out gives power (and
+ * optionally status), typically those of a preceding atg_bls_run.
+ * 1 <= k <= 64, distance >= 1. */
+int atg_bls_top_k(atg_ctx *ctx, const atg_bls_batch *batch,
+                  const atg_bls_out *out, int k, int distance,
+                  int64_t *index_out, int32_t *count_out);
 
 /* Average duration (ms) of the main kernel over the runs since the last
  * reset, timed with HIP events on the context stream; resets when
