@@ -26,12 +26,17 @@ ATG_F64 = 2
 OBJECTIVES = {"likelihood": 0, "snr": 1}
 MODES = {"parity": 0, "fast": 1}
 
+ATG_STATS_FIELDS = 16
+STATS_STATUS = {0: "ok", 1: "no point inside the transit box",
+                2: "more transits than max_transits"}
+
 # symbols declared in include/atg_bls.h
 EXPORTS = (
     "atg_abi_version", "atg_ctx_create", "atg_ctx_destroy",
     "atg_ctx_set_stream", "atg_last_error", "atg_ctx_set_tuning",
     "atg_ctx_get_tuning", "atg_bls_workspace_size", "atg_bls_run",
     "atg_bls_run_sync", "atg_timing_main_kernel", "atg_bls_top_k",
+    "atg_bls_stats", "atg_bls_model",
 )
 
 _vp = ctypes.c_void_p
@@ -103,6 +108,14 @@ def lib():
     L.atg_bls_top_k.argtypes = [_vp, ctypes.POINTER(Batch),
                                 ctypes.POINTER(Out), ctypes.c_int,
                                 ctypes.c_int, _vp, _vp]
+    L.atg_bls_stats.restype = ctypes.c_int
+    L.atg_bls_stats.argtypes = [_vp, ctypes.POINTER(Batch), ctypes.c_int64,
+                                _vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp,
+                                _vp, _vp]
+    L.atg_bls_model.restype = ctypes.c_int
+    L.atg_bls_model.argtypes = [_vp, ctypes.c_int64, _vp, ctypes.c_double,
+                                ctypes.c_double, ctypes.c_double,
+                                ctypes.c_double, ctypes.c_double, _vp, _vp]
     L.atg_timing_main_kernel.restype = ctypes.c_int
     L.atg_timing_main_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_double)]

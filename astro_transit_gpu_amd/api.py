@@ -194,3 +194,71 @@ class BoxLeastSquaresGPU:
             depth_snr=f["depth_snr"], log_likelihood=f["log_likelihood"],
             best_index=best,
         )
+
+    def compute_stats(self, period, duration, transit_time, device=None,
+                       max_transits=4096):
+        """astropy BoxLeastSquares.compute_stats (astropy timeseries/
+        periodograms/bls/core.py:427-570) for float inputs, on the GPU
+        (atg_bls_stats): depth, depth_phased, depth_half, depth_odd,
+        depth_even as (value, error), harmonic_amplitude,
+        harmonic_delta_log_likelihood, transit_times, per_transit_count,
+        per_transit_log_likelihood."""
+        period, duration = _check_candidate(period, duration)
+        ctx = default_context(device)
+        st = ctx.compute_stats(_vetting_batch(self, ctx), [0], period,
+                               duration, float(transit_time), max_transits)[0]
+        if st["status"] == 1:
+            raise ValueError("no data point lies inside the transit box")
+        if st["status"] == 2:
+            raise ValueError("more than %d transits in the baseline"
+                             % max_transits)
+        st.pop("status")
+        st.pop("model_levels")
+        return st
+
+    def model(self, t_model, period, duration, transit_time, device=None):
+        """astropy BoxLeastSquares.model (core.py:375-425) for float
+        inputs: the weighted mean flux inside and outside the box, evaluated
+        at t_model (atg_bls_stats for the levels, atg_bls_model for the
+        times)."""
+        period, duration = _check_candidate(period, duration)
+        ctx = default_context(device)
+        st = ctx.compute_stats(_vetting_batch(self, ctx), [0], period,
+                               duration, float(transit_time), 1)[0]
+        if st["status"] == 1:
+            raise ValueError("no data point lies inside the transit box")
+        y_in, y_out = st["model_levels"]
+        return ctx.model(_host(t_model), period, duration, transit_time, y_in,
+                         y_out)[1]
+
+    def transit_mask(self, t, period, duration, transit_time, device=None):
+        """astropy BoxLeastSquares.transit_mask (core.py:572-603) for float
+        inputs (atg_bls_model)."""
+        period, duration = _check_candidate(period, duration)
+        return default_context(device).model(_host(t), period, duration,
+                                             transit_time)[0]
+
+
+def _vetting_batch(obj, ctx):
+    t = _host(obj.t)
+    y = _host(obj.y)
+    dy = _host(obj.dy)
+    y32 = y.dtype == np.float32 and (dy is None or dy.dtype == np.float32)
+    return DeviceBatch.from_host([t], [y], None if dy is None else [dy],
+                                 np.array([1.0]), np.array([0.5]),
+                                 device=ctx.device,
+                                 y_dtype=np.float32 if y32 else np.float64)
+
+
+def _check_candidate(period, duration):
+    period = float(np.asarray(period))
+    duration = float(np.asarray(duration))
+    if period <= 0:
+        raise ValueError("All periods must be positive.")
+    if duration <= 0:
+        raise ValueError("All durations must be positive.")
+    if not period > duration:
+        raise ValueError("The maximum transit duration must be shorter "
+                         "than the minimum period")
+    return period, duration
+

@@ -1762,6 +1762,318 @@ __global__ __launch_bounds__(NT) void k_topk(const long long *per_off,
     if (tid == 0) cnt_out[lc] = count;
 }
 
+// ------------------------------------------------------ k_stats / k_model
+//
+// Vetting statistics of one (curve, period, duration, transit time)
+// candidate, astropy BoxLeastSquares.compute_stats (astropy timeseries/
+// periodograms/bls/core.py:427-570) on the raw t and y of the curve:
+// weighted levels inside and outside the box and of the odd, even,
+// half-period and half-phase boxes, the per-transit point counts and log
+// likelihoods, and a least-squares sinusoid at the period.  Sums are FP64
+// block reductions in a fixed order (deterministic); the per-transit sums
+// are taken in point order by one wave.
+
+constexpr int MAXTR = 4096;  // per-transit accumulators in LDS
+constexpr int NSUM = 25;     // FP64 sums of the first pass
+constexpr int NCNT = 8;      // point counts of the eight masks
+
+// numpy's float remainder (floored modulo via fmod)
+__device__ __forceinline__ double np_mod(double a, double b) {
+    double m = fmod(a, b);
+    if (m != 0.0) {
+        if ((b < 0.0) != (m < 0.0)) m += b;
+    } else {
+        m = copysign(0.0, b);
+    }
+    return m;
+}
+
+__device__ __forceinline__ bool in_box(double t, double T0, double P,
+                                       double d) {
+    const double hp = 0.5 * P;
+    return fabs(np_mod(t - T0 + hp, P) - hp) < 0.5 * d;
+}
+
+template <typename T>
+__device__ T block_sum_bcast(T v, T *scr) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) scr[threadIdx.x >> 6] = v;
+    __syncthreads();
+    T r = scr[0];
+    for (int k = 1; k < NW; ++k) r += scr[k];
+    return r;
+}
+
+// (level, variance) of a mask, or the depth below `ref` with its error;
+// (0, inf) when the mask is empty or the reference undefined
+__device__ void level_of(int cnt, double W, double Y, const double *ref,
+                         double &a, double &b) {
+    if (cnt > 0 && (ref == nullptr || isfinite(ref[1]))) {
+        const double var = 1.0 / W;
+        const double mean = Y * var;
+        if (!ref) {
+            a = mean;
+            b = var;
+        } else {
+            a = ref[0] - mean;
+            b = sqrt(var + ref[1]);
+        }
+        return;
+    }
+    a = 0.0;
+    b = INFINITY;
+}
+
+__global__ __launch_bounds__(NT) void k_stats(
+    Batch b, long long n_cand, const int *cand_lc, const double *cper,
+    const double *cdur, const double *ct0, int cap, double *stats,
+    int *pt_count, double *pt_ll, int *status) {
+    __shared__ double dsc[NW];
+    __shared__ long long lsc[NW];
+    __shared__ int isc[NW];
+    __shared__ double s_par[6];
+    __shared__ double ll_l[MAXTR];
+    __shared__ int cnt_l[MAXTR];
+    const long long c = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lc = cand_lc[c];
+    const long long off = b.lc_off[lc];
+    const int n = (int)(b.lc_off[lc + 1] - off);
+    const double P = cper[c], d = cdur[c], T0 = ct0[c];
+    const double hp = 0.5 * P, twoP = 2.0 * P, qP = 0.25 * P, hP = 0.5 * P;
+    const double twopi = 6.283185307179586;
+
+    // pass 1: weights and weighted sums of every mask, the sinusoid's
+    // normal equations, the range of transit numbers
+    double a[NSUM];
+    int cnt[NCNT];
+#pragma unroll
+    for (int k = 0; k < NSUM; ++k) a[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NCNT; ++k) cnt[k] = 0;
+    long long idmin = LLONG_MAX, idmax = LLONG_MIN;
+    for (int i = tid; i < n; i += NT) {
+        const double t = b.t[off + i];
+        const double y = load_val(b.y, b.y_dt, off + i);
+        double iv = 1.0;
+        if (b.dy) {
+            const double e = load_val(b.dy, b.dy_dt, off + i);
+            iv = 1.0 / (e * e);
+        }
+        const double yw = y * iv;
+        const bool m_in = fabs(np_mod(t - T0 + hp, P) - hp) < 0.5 * d;
+        const bool m_odd = fabs(np_mod(t - T0, twoP) - P) < 0.5 * d;
+        const bool m_even = fabs(np_mod(t - T0 + P, twoP) - P) < 0.5 * d;
+        const bool m_ph = fabs(np_mod(t - T0, P) - hp) < 0.5 * d;
+        const bool m_half = fabs(np_mod(t - T0 + qP, hP) - qP) < 0.5 * d;
+        const bool msk[NCNT] = {m_in,  !m_in, m_odd,  m_even,
+                                m_ph,  !m_ph && !m_in, m_half, !m_half};
+#pragma unroll
+        for (int k = 0; k < NCNT; ++k)
+            if (msk[k]) {
+                a[2 * k] += iv;
+                a[2 * k + 1] += yw;
+                cnt[k] += 1;
+            }
+        const double arg = twopi * t / P;
+        const double sn = sin(arg), cs = cos(arg);
+        a[16] += sn * sn * iv;
+        a[17] += sn * cs * iv;
+        a[18] += sn * iv;
+        a[19] += cs * cs * iv;
+        a[20] += cs * iv;
+        a[21] += iv;
+        a[22] += sn * yw;
+        a[23] += cs * yw;
+        a[24] += yw;
+        if (m_in) {
+            const long long id = llrint((t - T0) / P);
+            idmin = min(idmin, id);
+            idmax = max(idmax, id);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NSUM; ++k) a[k] = block_sum_bcast(a[k], dsc);
+#pragma unroll
+    for (int k = 0; k < NCNT; ++k) cnt[k] = block_sum_bcast(cnt[k], isc);
+    idmin = block_reduce(idmin, MinOp(), lsc);
+    __syncthreads();
+    idmax = block_reduce(idmax, MaxOp(), lsc);
+    double *so = stats + c * ATG_STATS_FIELDS;
+    if (cnt[0] == 0) {
+        if (tid < ATG_STATS_FIELDS) so[tid] = 0.0;
+        if (tid == 0) status[c] = ATG_STATS_NO_TRANSIT;
+        return;
+    }
+    if (tid == 0) {
+        double out[2], dep[2], odd[2], even[2], phref[2], ph[2], nh[2],
+            half[2];
+        level_of(cnt[1], a[2], a[3], nullptr, out[0], out[1]);
+        level_of(cnt[0], a[0], a[1], out, dep[0], dep[1]);
+        level_of(cnt[2], a[4], a[5], out, odd[0], odd[1]);
+        level_of(cnt[3], a[6], a[7], out, even[0], even[1]);
+        level_of(cnt[5], a[10], a[11], nullptr, phref[0], phref[1]);
+        level_of(cnt[4], a[8], a[9], phref, ph[0], ph[1]);
+        level_of(cnt[7], a[14], a[15], nullptr, nh[0], nh[1]);
+        level_of(cnt[6], a[12], a[13], nh, half[0], half[1]);
+        // sinusoid: 3x3 normal equations, Gaussian elimination with
+        // partial pivoting
+        double M[3][4] = {{a[16], a[17], a[18], a[22]},
+                          {a[17], a[19], a[20], a[23]},
+                          {a[18], a[20], a[21], a[24]}};
+        for (int col = 0; col < 3; ++col) {
+            int piv = col;
+            for (int r = col + 1; r < 3; ++r)
+                if (fabs(M[r][col]) > fabs(M[piv][col])) piv = r;
+            if (piv != col)
+                for (int k = 0; k < 4; ++k) {
+                    const double tmp = M[col][k];
+                    M[col][k] = M[piv][k];
+                    M[piv][k] = tmp;
+                }
+            for (int r = col + 1; r < 3; ++r) {
+                const double f = M[r][col] / M[col][col];
+                for (int k = col; k < 4; ++k) M[r][k] -= f * M[col][k];
+            }
+        }
+        double wv[3];
+        for (int r = 2; r >= 0; --r) {
+            double acc = M[r][3];
+            for (int k = r + 1; k < 3; ++k) acc -= M[r][k] * wv[k];
+            wv[r] = acc / M[r][r];
+        }
+        so[0] = dep[0];
+        so[1] = dep[1];
+        so[2] = ph[0];
+        so[3] = ph[1];
+        so[4] = half[0];
+        so[5] = half[1];
+        so[6] = odd[0];
+        so[7] = odd[1];
+        so[8] = even[0];
+        so[9] = even[1];
+        so[10] = sqrt(wv[0] * wv[0] + wv[1] * wv[1]);
+        so[12] = a[1] / a[0];  // model level inside (direct quotient)
+        so[13] = cnt[1] > 0 ? a[3] / a[2] : 0.0;
+        so[14] = (double)idmin;
+        so[15] = (double)(idmax - idmin + 1);
+        s_par[0] = out[0] - dep[0];  // y_in
+        s_par[1] = out[0];           // y_out
+        s_par[2] = wv[0];
+        s_par[3] = wv[1];
+        s_par[4] = wv[2];
+    }
+    __syncthreads();
+    const double y_in = s_par[0], y_out = s_par[1];
+    const double w0 = s_par[2], w1 = s_par[3], w2 = s_par[4];
+
+    // pass 2: the box and sinusoid log likelihoods
+    double sin_s = 0.0, in_s = 0.0, out_s = 0.0;
+    for (int i = tid; i < n; i += NT) {
+        const double t = b.t[off + i];
+        const double y = load_val(b.y, b.y_dt, off + i);
+        double iv = 1.0;
+        if (b.dy) {
+            const double e = load_val(b.dy, b.dy_dt, off + i);
+            iv = 1.0 / (e * e);
+        }
+        const double arg = twopi * t / P;
+        const double mod = sin(arg) * w0 + cos(arg) * w1 + w2;
+        const double r = y - mod;
+        sin_s += r * r * iv;
+        if (in_box(t, T0, P, d)) {
+            const double q = y - y_in;
+            in_s += iv * (q * q);
+        } else {
+            const double q = y - y_out;
+            out_s += iv * (q * q);
+        }
+    }
+    sin_s = block_sum_bcast(sin_s, dsc);
+    in_s = block_sum_bcast(in_s, dsc);
+    out_s = block_sum_bcast(out_s, dsc);
+    const long long n_tr = idmax - idmin + 1;
+    const bool fits = n_tr <= cap && n_tr <= MAXTR;
+    if (tid == 0) {
+        double full_ll = -0.5 * in_s;
+        full_ll -= 0.5 * out_s;
+        so[11] = -0.5 * sin_s - full_ll;
+        status[c] = fits ? ATG_STATS_OK : ATG_STATS_TOO_MANY_TRANSITS;
+    }
+    if (!fits) return;
+
+    // per-transit counts and log likelihoods, in point order (wave 0)
+    for (int k = tid; k < n_tr; k += NT) {
+        ll_l[k] = 0.0;
+        cnt_l[k] = 0;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        int cur = -1, ccur = 0;
+        double acc = 0.0;
+        for (int base = 0; base < n; base += 64) {
+            const int i = base + tid;
+            int id = -1;
+            double ll = 0.0;
+            if (i < n) {
+                const double t = b.t[off + i];
+                if (in_box(t, T0, P, d)) {
+                    const double y = load_val(b.y, b.y_dt, off + i);
+                    double iv = 1.0;
+                    if (b.dy) {
+                        const double e = load_val(b.dy, b.dy_dt, off + i);
+                        iv = 1.0 / (e * e);
+                    }
+                    const double qi = y - y_in, qo = y - y_out;
+                    ll = -0.5 * iv * (qi * qi - qo * qo);
+                    id = (int)(llrint((t - T0) / P) - idmin);
+                }
+            }
+            const int cntb = min(64, n - base);
+            for (int j = 0; j < cntb; ++j) {
+                const int idj = __builtin_amdgcn_readlane(id, j);
+                if (idj < 0) continue;
+                const double llj = readlane_d(ll, j);
+                if (idj != cur) {
+                    if (cur >= 0 && tid == 0) {
+                        ll_l[cur] = acc;
+                        cnt_l[cur] = ccur;
+                    }
+                    cur = idj;
+                    acc = ll_l[idj];
+                    ccur = cnt_l[idj];
+                }
+                acc += llj;
+                ccur += 1;
+            }
+        }
+        if (cur >= 0 && tid == 0) {
+            ll_l[cur] = acc;
+            cnt_l[cur] = ccur;
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < n_tr; k += NT) {
+        pt_count[c * cap + k] = cnt_l[k];
+        pt_ll[c * cap + k] = ll_l[k];
+    }
+}
+
+// astropy's transit_mask (core.py:572-603) and model (core.py:375-425)
+// evaluated at arbitrary times, given the model's two levels.
+__global__ __launch_bounds__(NT) void k_model(long long n, const double *t,
+                                              double P, double d, double T0,
+                                              double y_in, double y_out,
+                                              unsigned char *mask,
+                                              double *y_model) {
+    const long long i = (long long)blockIdx.x * NT + threadIdx.x;
+    if (i >= n) return;
+    const bool m = in_box(t[i], T0, P, d);
+    if (mask) mask[i] = m ? 1 : 0;
+    if (y_model) y_model[i] = m ? y_in : y_out;
+}
+
 // ----------------------------------------------------------------- k_final
 
 __global__ __launch_bounds__(NT) void k_final(Batch b, Out o, Work w) {
@@ -2173,6 +2485,61 @@ int atg_bls_top_k(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
                        (int *)(base + L.tk_pos),
                        (unsigned char *)(base + L.tk_st), k, distance,
                        (long long *)index_out, (int *)count_out);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(c, e, "kernel launch");
+    return ATG_OK;
+}
+
+int atg_bls_stats(atg_ctx *c, const atg_bls_batch *b, int64_t n_cand,
+                  const int32_t *cand_lc, const double *period,
+                  const double *duration, const double *transit_time,
+                  int max_transits, double *stats,
+                  int32_t *per_transit_count, double *per_transit_ll,
+                  int32_t *status) {
+    if (!c) return ATG_ERR_ARG;
+    if (!b || !b->t || !b->y || !b->lc_offsets)
+        return fail(c, ATG_ERR_ARG, "null batch, t, y or lc_offsets");
+    if (b->n_lc < 1 || b->n_lc > INT_MAX)
+        return fail(c, ATG_ERR_ARG, "n_lc must be in [1, 2^31)");
+    if (b->y_dtype != ATG_F32 && b->y_dtype != ATG_F64)
+        return fail(c, ATG_ERR_ARG, "y_dtype must be ATG_F32 or ATG_F64");
+    if (b->dy && b->dy_dtype != ATG_F32 && b->dy_dtype != ATG_F64)
+        return fail(c, ATG_ERR_ARG, "dy_dtype must be ATG_F32 or ATG_F64");
+    if (n_cand < 1 || n_cand > INT_MAX)
+        return fail(c, ATG_ERR_ARG, "n_cand must be in [1, 2^31)");
+    if (!cand_lc || !period || !duration || !transit_time || !stats ||
+        !per_transit_count || !per_transit_ll || !status)
+        return fail(c, ATG_ERR_ARG, "null candidate or output pointer");
+    if (max_transits < 1 || max_transits > MAXTR)
+        return fail(c, ATG_ERR_ARG, "max_transits must be in [1, " +
+                                        std::to_string(MAXTR) + "]");
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+    hipLaunchKernelGGL(k_stats, dim3((unsigned)n_cand), dim3(NT), 0,
+                       c->stream, to_batch(b), (long long)n_cand,
+                       (const int *)cand_lc, period, duration, transit_time,
+                       max_transits, stats, (int *)per_transit_count,
+                       per_transit_ll, (int *)status);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(c, e, "kernel launch");
+    return ATG_OK;
+}
+
+int atg_bls_model(atg_ctx *c, int64_t n, const double *t, double period,
+                  double duration, double transit_time, double y_in,
+                  double y_out, uint8_t *mask, double *y_model) {
+    if (!c) return ATG_ERR_ARG;
+    if (n < 0 || (n > 0 && !t) || (!mask && !y_model))
+        return fail(c, ATG_ERR_ARG, "bad size or null pointer");
+    if (n == 0) return ATG_OK;
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+    const long long blocks = (n + NT - 1) / NT;
+    if (blocks > INT_MAX) return fail(c, ATG_ERR_ARG, "too many points");
+    hipLaunchKernelGGL(k_model, dim3((unsigned)blocks), dim3(NT), 0,
+                       c->stream, (long long)n, t, period, duration,
+                       transit_time, y_in, y_out, (unsigned char *)mask,
+                       y_model);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(c, e, "kernel launch");
     return ATG_OK;

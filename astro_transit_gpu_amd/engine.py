@@ -321,6 +321,85 @@ class Context:
         return [[dict(zip(keys, map(float, vals[c, r])))
                  for r in range(int(cnt[c]))] for c in range(batch.n_lc)]
 
+    def compute_stats(self, batch: DeviceBatch, curve, period, duration,
+                      transit_time, max_transits=1024):
+        """Vetting statistics of candidates (atg_bls_stats; astropy
+        BoxLeastSquares.compute_stats per candidate).  ``curve``, ``period``,
+        ``duration``, ``transit_time``: one value per candidate.  Returns a
+        list of dicts with astropy's keys, plus ``status``."""
+        cl = np.atleast_1d(np.asarray(curve, dtype=np.int32))
+        n = cl.size
+        if n == 0:
+            return []
+        if cl.min() < 0 or cl.max() >= batch.n_lc:
+            raise ValueError("candidate curve index out of range")
+        vals = [np.array(np.broadcast_to(np.asarray(v, dtype=np.float64),
+                                         (n,)))
+                for v in (period, duration, transit_time)]
+        if not (np.all(vals[0] > 0) and np.all(vals[1] > 0)):
+            raise ValueError("periods and durations must be positive")
+        if np.any(vals[1] >= vals[0]):
+            raise ValueError("The maximum transit duration must be shorter "
+                             "than the minimum period")
+        dev = self.device
+        up = [torch.as_tensor(np.ascontiguousarray(v), device=dev)
+              for v in [cl] + vals]
+        cap = int(max_transits)
+        st = torch.empty((n, _abi.ATG_STATS_FIELDS), dtype=torch.float64,
+                         device=dev)
+        pc = torch.zeros((n, cap), dtype=torch.int32, device=dev)
+        pl = torch.zeros((n, cap), dtype=torch.float64, device=dev)
+        status = torch.empty(n, dtype=torch.int32, device=dev)
+        b, _ = self._structs(batch, DeviceResult.empty(
+            1, 1, batch.host_period_offsets, dev), "likelihood", "parity",
+            10)
+        self._check(self._lib.atg_bls_stats(
+            self._h, ctypes.byref(b), n, *[_ptr(u) for u in up], cap,
+            _ptr(st), _ptr(pc), _ptr(pl), _ptr(status)))
+        self.stream.synchronize()
+        st, pc, pl, status = (st.cpu().numpy(), pc.cpu().numpy(),
+                              pl.cpu().numpy(), status.cpu().numpy())
+        out = []
+        for i in range(n):
+            f = st[i]
+            n_tr = int(f[15]) if status[i] != 1 else 0
+            P, T0 = vals[0][i], vals[2][i]
+            full = status[i] == 0
+            out.append(dict(
+                transit_times=P * np.arange(int(f[14]), int(f[14]) + n_tr)
+                + T0,
+                per_transit_count=pc[i, :n_tr].astype(np.int64) if full
+                else None,
+                per_transit_log_likelihood=pl[i, :n_tr].copy() if full
+                else None,
+                depth=(f[0], f[1]), depth_phased=(f[2], f[3]),
+                depth_half=(f[4], f[5]), depth_odd=(f[6], f[7]),
+                depth_even=(f[8], f[9]), harmonic_amplitude=float(f[10]),
+                harmonic_delta_log_likelihood=float(f[11]),
+                model_levels=(float(f[12]), float(f[13])),
+                status=int(status[i])))
+        return out
+
+    def model(self, t, period, duration, transit_time, y_in=None,
+              y_out=None):
+        """Transit mask (and, given the two levels, the box model) at the
+        times ``t`` (atg_bls_model).  Returns (mask, y_model or None) as
+        numpy arrays."""
+        dev = self.device
+        tt = torch.as_tensor(np.ascontiguousarray(t, dtype=np.float64),
+                             device=dev)
+        n = int(tt.numel())
+        mask = torch.empty(n, dtype=torch.uint8, device=dev)
+        ym = None if y_in is None else torch.empty(n, dtype=torch.float64,
+                                                    device=dev)
+        self._check(self._lib.atg_bls_model(
+            self._h, n, _ptr(tt), float(period), float(duration),
+            float(transit_time), float(y_in or 0.0), float(y_out or 0.0),
+            _ptr(mask), _ptr(ym)))
+        self.stream.synchronize()
+        return (mask.cpu().numpy().astype(bool),
+                None if ym is None else ym.cpu().numpy())
+
     def workspace_bytes(self, batch: DeviceBatch):
         b, _ = self._structs(batch, DeviceResult.empty(
             1, 1, batch.host_period_offsets, self.device), "likelihood",

@@ -372,6 +372,35 @@ def main():
         line["fast"] = {"value": round(args.n_lc * max(2, args.steps // 2) /
                                        el_f, 2), "unit": "LC/s",
                         "kernel_ms": round(kms_f, 3)}
+        # top-5 candidates of every curve: device top-k vs the reference's
+        # host rule (scipy find_peaks over each periodogram)
+        from astro_transit_gpu_amd import get_top_k_candidates
+        ctx.run(batch, objective=objective, mode="parity", out=out)
+        ctx.top_k(batch, out, 5, 10)
+        torch.cuda.synchronize()
+        s = time.perf_counter()
+        idx, cnt = ctx.top_k(batch, out, 5, 10)
+        dev_ms = (time.perf_counter() - s) * 1e3
+        hp = out.to_host()
+        po = batch.host_period_offsets
+        s = time.perf_counter()
+        host_idx = []
+        for i in range(args.n_lc):
+            a, z = po[i], po[i + 1]
+            host_idx.append([int(c["period"]) for c in get_top_k_candidates(
+                dict(power=hp["power"][a:z], periods=np.arange(z - a),
+                     all_t0s=hp["transit_time"][a:z],
+                     all_durs=hp["duration"][a:z],
+                     all_depths=hp["depth"][a:z]), k=5)])
+        host_ms = (time.perf_counter() - s) * 1e3
+        hidx, hcnt = idx.cpu().numpy(), cnt.cpu().numpy()
+        same = sum(int(host_idx[i] == list(hidx[i, :int(hcnt[i])]))
+                   for i in range(args.n_lc))
+        line["top_k"] = {"k": 5, "distance": 10,
+                         "device_ms": round(dev_ms, 3),
+                         "host_scipy_ms": round(host_ms, 1),
+                         "curves_identical_to_host_rule": same,
+                         "curves": args.n_lc}
         # plain form: one curve per call, host arrays in and out
         lat = []
         for i in range(args.plain + 1):

@@ -179,6 +179,47 @@ int atg_bls_top_k(atg_ctx *ctx, const atg_bls_batch *batch,
                   const atg_bls_out *out, int k, int distance,
                   int64_t *index_out, int32_t *count_out);
 
+/* Vetting statistics (see atg_bls_stats). */
+#define ATG_STATS_FIELDS 16
+#define ATG_STATS_OK 0
+#define ATG_STATS_NO_TRANSIT 1        /* no point inside the box          */
+#define ATG_STATS_TOO_MANY_TRANSITS 2 /* per-transit arrays not written   */
+
+/* Vetting statistics of candidates (asynchronous on the context stream).
+ * Replaces astropy BoxLeastSquares.compute_stats (astropy timeseries/
+ * periodograms/bls/core.py:427-570), which the reference's vetting calls
+ * on its BLS candidates, for n_cand candidates at once: candidate i
+ * belongs to curve cand_lc[i] of `batch` (t, y, dy and lc_offsets are
+ * used; raw t and y, as astropy) and has period[i], duration[i] and the
+ * mid-transit time transit_time[i] (same time origin as t).
+ * stats[i*ATG_STATS_FIELDS + f], f =
+ *   0 depth, 1 its error, 2 depth_phased, 3 error, 4 depth_half,
+ *   5 error, 6 depth_odd, 7 error, 8 depth_even, 9 error,
+ *   10 harmonic_amplitude, 11 harmonic_delta_log_likelihood,
+ *   12 weighted mean inside the box, 13 weighted mean outside (the two
+ *   levels of astropy's model), 14 number of the first transit
+ *   (round((t - T0)/P) of the earliest in-transit point), 15 number of
+ *   transits n_tr (transit_times = T0 + P*(first + 0..n_tr-1)).
+ * per_transit_count / per_transit_ll[i*max_transits + j], j < n_tr: points
+ * and log likelihood of transit j.  status[i]: ATG_STATS_*.
+ * 1 <= max_transits <= 4096.  All pointers are device pointers. */
+int atg_bls_stats(atg_ctx *ctx, const atg_bls_batch *batch, int64_t n_cand,
+                  const int32_t *cand_lc, const double *period,
+                  const double *duration, const double *transit_time,
+                  int max_transits, double *stats,
+                  int32_t *per_transit_count, double *per_transit_ll,
+                  int32_t *status);
+
+/* Transit mask and box model at n times t (device pointers; either output
+ * may be NULL): mask[i] = 1 inside the box of (period, duration,
+ * transit_time), y_model[i] = y_in inside, y_out outside.  Replaces astropy
+ * BoxLeastSquares.transit_mask (core.py:572-603) and, with y_in / y_out
+ * from atg_bls_stats fields 12 / 13, BoxLeastSquares.model
+ * (core.py:375-425). */
+int atg_bls_model(atg_ctx *ctx, int64_t n, const double *t, double period,
+                  double duration, double transit_time, double y_in,
+                  double y_out, uint8_t *mask, double *y_model);
+
 /* Average duration (ms) of the main kernel over the runs since the last
  * reset, timed with HIP events on the context stream; resets when
  * `reset` != 0.  Returns the number of timed runs (0 = none). */

@@ -11,6 +11,10 @@ The product (astro_transit_gpu_amd/) never imports it.
 * :func:`bls_slow` restates astropy's brute-force ``bls_slow``
   (bls/methods.py:98-143) in numpy for tiny inputs.  It is an independent
   definition of every output field, so fast == slow is an invariant check.
+* :func:`compute_stats`, :func:`model`, :func:`transit_mask` restate
+  astropy's vetting helpers (bls/core.py:375-603) in numpy.
+* :func:`power_variant` evaluates single-precision alternatives of the
+  algorithm for the precision study (see atg_oracle_bls_variant).
 
 Pinning (every function here has at least one):
   - ``power``/``bls_impl``: bit-for-bit against the outputs of astropy's own
@@ -19,6 +23,13 @@ Pinning (every function here has at least one):
   - ``bls_slow``: astropy's test_fast_method invariant (test_bls.py:414-428,
     fast == slow on all fields) on the astropy ``data`` fixture, plus the
     closed forms loglike = W_in*depth^2/2 and snr = depth/depth_err.
+  - ``compute_stats`` / ``model`` / ``transit_mask``: astropy's
+    test_compute_stats and test_model known answers (test_bls.py:546-658:
+    per-transit counts [9, 7, 7, 7, 8], harmonic amplitude and delta log
+    likelihood with and without dy, agreement with the oversample-1000
+    periodogram, the model equal to a two-level linear regression).
+  - ``power_variant``: equal to ``power`` bit for bit with both switches
+    off (its FP32 arithmetic is what the study measures).
 """
 
 import ctypes
@@ -183,6 +194,90 @@ def bls_slow(t, y, dy, periods, durations, objective="likelihood",
     out = dict(zip(FIELDS, map(np.array, zip(*rows))))
     out["transit_time"] = out["transit_time"] + t_ref
     return out
+
+
+def _box_mask(t, period, duration, transit_time, phase_shift=0.5):
+    hp = phase_shift * period
+    return np.abs((t - transit_time + hp) % period - hp) < 0.5 * duration
+
+
+def transit_mask(t, period, duration, transit_time):
+    """astropy BoxLeastSquares.transit_mask (core.py:572-603) for float
+    inputs: |(t - T0 + P/2) mod P - P/2| < duration/2."""
+    return _box_mask(np.asarray(t, dtype=np.float64), float(period),
+                     float(duration), float(transit_time))
+
+
+def model(t, y, dy, t_model, period, duration, transit_time):
+    """astropy BoxLeastSquares.model (core.py:375-425) for float inputs:
+    the weighted means inside and outside the transit mask of the data,
+    evaluated on t_model."""
+    t = np.asarray(t, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    ivar = np.ones_like(y) if dy is None else \
+        1.0 / np.asarray(dy, dtype=np.float64) ** 2
+    m_in = _box_mask(t, period, duration, transit_time)
+    y_in = np.sum(y[m_in] * ivar[m_in]) / np.sum(ivar[m_in])
+    y_out = np.sum(y[~m_in] * ivar[~m_in]) / np.sum(ivar[~m_in])
+    t_model = np.asarray(t_model, dtype=np.float64)
+    out = np.full(t_model.shape, y_out)
+    out[_box_mask(t_model, period, duration, transit_time)] = y_in
+    return out
+
+
+def compute_stats(t, y, dy, period, duration, transit_time):
+    """astropy BoxLeastSquares.compute_stats (core.py:427-570) for float
+    inputs (raw t and y; t is not shifted by min(t) there)."""
+    t = np.asarray(t, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    ivar = np.ones_like(y) if dy is None else \
+        1.0 / np.asarray(dy, dtype=np.float64) ** 2
+    P, d, T0 = float(period), float(duration), float(transit_time)
+
+    def level(m, ref=None):
+        # weighted mean of the masked points, or the depth below ``ref``
+        # (level, variance) with its uncertainty; (0, inf) when undefined
+        if np.any(m) and (ref is None or np.isfinite(ref[1])):
+            var = 1.0 / np.sum(ivar[m])
+            mean = np.sum(y[m] * ivar[m]) * var
+            if ref is None:
+                return mean, var
+            return ref[0] - mean, np.sqrt(var + ref[1])
+        return 0.0, np.inf
+
+    m_in = _box_mask(t, P, d, T0)
+    m_out = ~m_in
+    m_odd = np.abs((t - T0) % (2 * P) - P) < 0.5 * d
+    m_even = np.abs((t - T0 + P) % (2 * P) - P) < 0.5 * d
+    out = level(m_out)
+    depth = level(m_in, out)
+    depth_odd = level(m_odd, out)
+    depth_even = level(m_even, out)
+    y_out = out[0]
+    y_in = y_out - depth[0]
+    m_phase = np.abs((t - T0) % P - 0.5 * P) < 0.5 * d
+    depth_phased = level(m_phase, level(~m_phase & m_out))
+    m_half = np.abs((t - T0 + 0.25 * P) % (0.5 * P) - 0.25 * P) < 0.5 * d
+    depth_half = level(m_half, level(~m_half))
+
+    ids = np.round((t[m_in] - T0) / P).astype(np.int64)
+    first = ids.min()
+    n_tr = ids.max() - first + 1
+    counts = np.bincount(ids - first, minlength=n_tr)
+    ll = -0.5 * ivar[m_in] * ((y[m_in] - y_in) ** 2 - (y[m_in] - y_out) ** 2)
+    lls = np.array([np.sum(ll[ids - first == i]) for i in range(n_tr)])
+    full_ll = -0.5 * np.sum(ivar[m_in] * (y[m_in] - y_in) ** 2)
+    full_ll -= 0.5 * np.sum(ivar[m_out] * (y[m_out] - y_out) ** 2)
+    A = np.vstack((np.sin(2 * np.pi * t / P), np.cos(2 * np.pi * t / P),
+                   np.ones_like(t))).T
+    w = np.linalg.solve(A.T @ (A * ivar[:, None]), A.T @ (y * ivar))
+    sin_ll = -0.5 * np.sum((y - A @ w) ** 2 * ivar)
+    return dict(transit_times=P * np.arange(first, first + n_tr) + T0,
+                per_transit_count=counts, per_transit_log_likelihood=lls,
+                depth=depth, depth_phased=depth_phased, depth_half=depth_half,
+                depth_odd=depth_odd, depth_even=depth_even,
+                harmonic_amplitude=float(np.sqrt(np.sum(w[:2] ** 2))),
+                harmonic_delta_log_likelihood=float(sin_ll - full_ll))
 
 
 def load_golden(path):
