@@ -63,6 +63,8 @@ struct LcParams {
     int n;            // points
     int status;       // ATG_LC_*
     int n_dur;
+    double tol_r;     // near-integer tolerance of t'/P (uniform bound)
+    double tol_x;     // near-integer tolerance of phase/bd (uniform bound)
     double seq_y;     // parity mode: the oracle's totals, summed in FP64
     double seq_iv;    // in point order (sum of y*ivar, sum of ivar)
     int sorted;       // t non-decreasing (enables the ranged phase tiles)
@@ -210,25 +212,6 @@ __device__ __forceinline__ int fold_bin(double t, double P, double invP,
     return (int)xi + 1;
 }
 
-// Branch-free form of fold_bin for the hot loop: the bin from the rounded
-// reciprocals, plus a flag raised when either quotient lies within 2^-50
-// (relative) of an integer, the only case where the result can differ from
-// the oracle's divisions.  Flagged points are re-binned with fold_exact.
-__device__ __forceinline__ int fold_fast(double t, double P, double invP,
-                                         double inv_bd, bool &flag) {
-    const double r = t * invP;
-    const double q = floor(r);
-    const double fr = r - q;
-    const double tol = r * 0x1p-50;
-    const double ph = t - P * q;
-    const double x = ph * inv_bd;
-    const double xi = trunc(x);
-    const double fx = x - xi;
-    const double tolx = fabs(x) * 0x1p-50;
-    flag = (fr <= tol) | (fr >= 1.0 - tol) | (fx <= tolx) | (fx >= 1.0 - tolx);
-    return (int)xi + 1;
-}
-
 // The oracle's bin rule with true divisions.
 __device__ __forceinline__ int fold_exact(double t, double P, double bd) {
     const double ph = t - P * floor(t / P);
@@ -282,15 +265,19 @@ __device__ int lower_key(const double *tn, int n, double P, double invP,
 // Bins every point of the lane for C periods at once.  C is a compile-time
 // constant so that the C fold chains are independent straight-line code the
 // scheduler can interleave; the next point's loads are issued before the
-// current point is folded.  Bins that a near-integer quotient makes
-// uncertain are recomputed with true divisions before the atomics.
+// current point is folded.  The near-integer test uses the curve's uniform
+// tolerances (tol_r, tol_x: bounds of the per-point ones of fold_bin), so a
+// superset of the uncertain bins is recomputed with true divisions before
+// the atomics and every other bin is known to lie in [1, nb - os].
 template <int C, bool EXACT>
 __device__ __forceinline__ void bin_points(
     const double *tp, const long long *wp, const long long *yp, int L,
     const double *per, const double *inv, const int *nb, const int *base,
-    double bd, double inv_bd, long long *binW, long long *binY) {
+    double bd, double inv_bd, double tol_r, double tol_x, long long *binW,
+    long long *binY) {
     double tv_n = tp[0];
     long long wv_n = wp[0], yv_n = yp[0];
+    const double hr = 1.0 - tol_r, hx = 1.0 - tol_x;
     for (int i = 0; i < L; ++i) {
         const double tv = tv_n;
         const long long wv = wv_n, yv = yv_n;
@@ -299,32 +286,45 @@ __device__ __forceinline__ void bin_points(
             wv_n = wp[(long long)(i + 1) * NT];
             yv_n = yp[(long long)(i + 1) * NT];
         }
-        int ind[C];
-        unsigned flags = 0;
+        int at[C];
+        bool fl[C];
+        bool any = false;
 #pragma unroll
         for (int s = 0; s < C; ++s) {
             if (EXACT) {
-                bool f;
-                ind[s] = fold_fast(tv, per[s], inv[s], inv_bd, f);
-                flags |= (unsigned)f << s;
+                // bin from the rounded reciprocals; a quotient within the
+                // uniform tolerance of an integer is flagged for the
+                // oracle's divisions.  Unflagged bins lie in [1, nb - os].
+                const double r = tv * inv[s];
+                const double q = floor(r);
+                const double fr = r - q;
+                const double ph = tv - per[s] * q;
+                const double x = ph * inv_bd;
+                const double xi = trunc(x);
+                const double fx = x - xi;
+                fl[s] = (fr <= tol_r) | (fr >= hr) | (fx <= tol_x) |
+                        (fx >= hx);
+                any |= fl[s];
+                at[s] = base[s] + 1 + (int)xi;
             } else {
                 // fast mode: FMA phase and rounded reciprocals only; a point
                 // within ~1e-15 of a bin edge may land in the neighbour bin
                 const double ph = fma(-per[s], floor(tv * inv[s]), tv);
-                ind[s] = (int)(ph * inv_bd) + 1;
+                at[s] = base[s] + min(max((int)(ph * inv_bd) + 1, 0), nb[s]);
             }
         }
-        if (EXACT && flags) {
+        if (EXACT && any) {
 #pragma unroll
             for (int s = 0; s < C; ++s)
-                if ((flags >> s) & 1u) ind[s] = fold_exact(tv, per[s], bd);
+                if (fl[s])
+                    at[s] = base[s] +
+                            min(max(fold_exact(tv, per[s], bd), 0), nb[s]);
         }
 #pragma unroll
         for (int s = 0; s < C; ++s) {
-            const int at = base[s] + min(max(ind[s], 0), nb[s]);
-            __hip_atomic_fetch_add(&binW[at], wv, __ATOMIC_RELAXED,
+            __hip_atomic_fetch_add(&binW[at[s]], wv, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(&binY[at], yv, __ATOMIC_RELAXED,
+            __hip_atomic_fetch_add(&binY[at[s]], yv, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
@@ -453,10 +453,11 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     }
 
     // pass A: min(t), NaN and dy checks, time order
-    double tmin = INFINITY;
+    double tmin = INFINITY, tmax = -INFINITY;
     int bad = 0;
     for (int i = tid; i < n; i += NT) {
         double tv = b.t[off + i];
+        tmax = fmax(tmax, tv);
         double yv = load_val(b.y, b.y_dt, off + i);
         if (isnan(tv) || isnan(yv)) bad |= 1;
         if (i > 0 && !(tv >= b.t[off + i - 1])) bad |= 4;
@@ -467,6 +468,7 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
         }
     }
     tmin = block_reduce(tmin, MinOp(), dscr);
+    tmax = block_reduce(tmax, MaxOp(), dscr);
     bad = block_reduce(bad, OrOp(), iscr);
     if (status == ATG_LC_OK) {
         if (bad & 1) status = ATG_LC_NAN;
@@ -535,6 +537,11 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     P.w_unit = ldexp(1.0, -sw);
     P.y_unit = ldexp(1.0, -sy);
     P.bd = dmin / (double)b.oversample;
+    // bounds of |t'/P| and |phase/bd| over the curve; a product-based
+    // quotient can change floor() only within 8 ulp of an integer, i.e.
+    // within 2^-50 of the bound, so 2^-49 flags a superset of those points
+    P.tol_r = ldexp((tmax - tmin) / pmin + 1.0, -49);
+    P.tol_x = ldexp(ceil(pmax / P.bd) + b.oversample + 2.0, -49);
 
     // pass C: lane-major workspace; lane j owns points [j*L, (j+1)*L)
     long long wq_sum = 0, yq_sum = 0;
@@ -1179,7 +1186,7 @@ __global__ __launch_bounds__(NT) void k_main(Batch b, Out o, Work w,
 #define ATG_BIN(C, E)                                                       \
     case C:                                                                 \
         bin_points<C, E>(tp, wp, yp, L, per, inv, nb, base, bd, inv_bd,     \
-                         binW, binY);                                       \
+                         P.tol_r, P.tol_x, binW, binY);                     \
         break;
                 if (b.mode == ATG_MODE_FAST) {
                     switch (cnt) {
