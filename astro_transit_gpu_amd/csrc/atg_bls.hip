@@ -70,6 +70,9 @@ struct LcParams {
     int sorted;       // t non-decreasing (enables the ranged phase tiles)
     int lc_self;      // index of the curve
     int natural;      // natural-order copies of the points were written
+    int packed;       // uniform weights: one packed int64 per bin
+    int ybits;        // packed bin = count << ybits + weight*flux quantum
+    long long wq_one; // the common quantized weight of a packed curve
 };
 
 struct Batch {
@@ -101,7 +104,8 @@ struct Work {
     unsigned int *qnext;  // [NQ] queue heads
     unsigned int *counter;
     double *ws_t;         // lane-major t'
-    long long *ws_w;      // lane-major quantized ivar
+    long long *ws_w;      // lane-major quantized ivar (packed curves: the
+                          // packed bin increment 1 << ybits + ws_y)
     long long *ws_y;      // lane-major quantized ivar * (y - median)
     double *nat_t;        // the same three arrays in point order, written
     long long *nat_w;     // only for time-sorted curves with periods whose
@@ -269,14 +273,14 @@ __device__ int lower_key(const double *tn, int n, double P, double invP,
 // tolerances (tol_r, tol_x: bounds of the per-point ones of fold_bin), so a
 // superset of the uncertain bins is recomputed with true divisions before
 // the atomics and every other bin is known to lie in [1, nb - os].
-template <int C, bool EXACT>
+template <int C, bool EXACT, bool PACKED>
 __device__ __forceinline__ void bin_points(
     const double *tp, const long long *wp, const long long *yp, int L,
     const double *per, const double *inv, const int *nb, const int *base,
     double bd, double inv_bd, double tol_r, double tol_x, long long *binW,
     long long *binY) {
     double tv_n = tp[0];
-    long long wv_n = wp[0], yv_n = yp[0];
+    long long wv_n = wp[0], yv_n = PACKED ? 0 : yp[0];
     const double hr = 1.0 - tol_r, hx = 1.0 - tol_x;
     for (int i = 0; i < L; ++i) {
         const double tv = tv_n;
@@ -284,7 +288,7 @@ __device__ __forceinline__ void bin_points(
         if (i + 1 < L) {
             tv_n = tp[(long long)(i + 1) * NT];
             wv_n = wp[(long long)(i + 1) * NT];
-            yv_n = yp[(long long)(i + 1) * NT];
+            if (!PACKED) yv_n = yp[(long long)(i + 1) * NT];
         }
         int at[C];
         bool fl[C];
@@ -324,10 +328,16 @@ __device__ __forceinline__ void bin_points(
         for (int s = 0; s < C; ++s) {
             __hip_atomic_fetch_add(&binW[at[s]], wv, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(&binY[at[s]], yv, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (!PACKED)
+                __hip_atomic_fetch_add(&binY[at[s]], yv, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
+}
+
+// Sign-extended weight*flux field and point count of a packed bin value.
+__device__ __forceinline__ long long packed_y(long long v, int ybits) {
+    return (long long)((unsigned long long)v << (64 - ybits)) >> (64 - ybits);
 }
 
 // ------------------------------------------------------------------ k_prep
@@ -513,8 +523,8 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
         P.seq_iv = si;
     }
 
-    // pass B: magnitudes for the fixed-point scales
-    double sw_sum = 0.0, sy_sum = 0.0;
+    // pass B: magnitudes for the fixed-point scales, weight uniformity
+    double sw_sum = 0.0, sy_sum = 0.0, iv_min = INFINITY, iv_max = -INFINITY;
     for (int i = tid; i < n; i += NT) {
         double iv = 1.0;
         if (b.dy) {
@@ -524,13 +534,25 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
         double yc = load_val(b.y, b.y_dt, off + i) - med;
         sw_sum += iv;
         sy_sum += fabs(yc * iv);
+        iv_min = fmin(iv_min, iv);
+        iv_max = fmax(iv_max, iv);
     }
     sw_sum = block_reduce(sw_sum, AddOp<double>(), dscr);
     sy_sum = block_reduce(sy_sum, AddOp<double>(), dscr);
-    // scale so that the sum of |quantized| values stays below 2^60; the
-    // padding copies at most double a prefix sum, well inside int64.
+    iv_min = block_reduce(iv_min, MinOp(), dscr);
+    iv_max = block_reduce(iv_max, MaxOp(), dscr);
+    // Uniform weights: a bin is (point count, weight*flux sum) packed into
+    // one int64 as count << ybits + sum.  Prefix sums (with the padding
+    // copies) reach at most 2n points, which takes cbits bits; the flux
+    // field keeps 63 - cbits bits and its scale leaves 2 bits of headroom.
+    const int cbits = ilogb(2.0 * n + 1.0) + 1;
+    P.packed = (iv_min == iv_max && 63 - cbits >= 40) ? 1 : 0;
+    P.ybits = 63 - cbits;
+    // scale so that the sum of |quantized| values stays below 2^60 (packed:
+    // 2^(ybits-3)); the padding copies at most double a prefix sum.
+    const int ytop = P.packed ? P.ybits - 4 : 59;
     int sw = 59 - ilogb(sw_sum);
-    int sy = (sy_sum > 0.0 && isfinite(sy_sum)) ? 59 - ilogb(sy_sum) : 0;
+    int sy = (sy_sum > 0.0 && isfinite(sy_sum)) ? ytop - ilogb(sy_sum) : 0;
     sw = max(-1000, min(1000, sw));
     sy = max(-1000, min(1000, sy));
     const double wscale = ldexp(1.0, sw), yscale = ldexp(1.0, sy);
@@ -554,8 +576,11 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
         if (src < n) quantize(b, off + s, med, wscale, yscale, wq, yq);
         long long slot = P.ws_off + (long long)i * NT + tid;
         w.ws_t[slot] = tp;
-        w.ws_w[slot] = wq;
+        // packed curves keep the packed bin increment in ws_w
+        w.ws_w[slot] = P.packed ? (src < n ? (1ll << P.ybits) : 0ll) + yq
+                                : wq;
         w.ws_y[slot] = yq;
+        if (src == 0) P.wq_one = wq;
         wq_sum += wq;
         yq_sum += yq;
     }
@@ -666,7 +691,10 @@ __device__ __forceinline__ void write_nan(const Out &o, long long p) {
 }
 
 // Padding, prefix scan and box scan of one period whose int64 histogram
-// occupies bins [base, base + nb] of binW / binY; writes period p.
+// occupies bins [base, base + nb] of binW / binY; writes period p.  A
+// packed histogram (P.packed) lives in binW alone; the caller then passes
+// binY = binW + nb + 1, the free space above it that receives the FP64
+// cumulative flux.
 __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
                               long long p, double period, int nb, int base,
                               long long *binW, long long *binY,
@@ -674,16 +702,17 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
                               double *dscr, int *iscr) {
     const int tid = threadIdx.x;
     const int os = b.oversample;
+    const bool packed = P.packed;
     // padding: bins[nb-os+j-1] = bins[j], j = 1..os (an assignment)
     if (nb - os > os) {
         for (int j = 1 + tid; j <= os; j += NT) {
             binW[base + nb - os + j - 1] = binW[base + j];
-            binY[base + nb - os + j - 1] = binY[base + j];
+            if (!packed) binY[base + nb - os + j - 1] = binY[base + j];
         }
     } else if (tid == 0) {  // overlapping ranges: the oracle's order
         for (int j = 1, d = nb - os; j <= os; ++j, ++d) {
             binW[base + d] = binW[base + j];
-            binY[base + d] = binY[base + j];
+            if (!packed) binY[base + d] = binY[base + j];
         }
     }
     __syncthreads();
@@ -695,20 +724,29 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
     long long sw = 0, sy = 0;
     for (int i = b0; i < b1; ++i) {
         sw += binW[base + i];
-        sy += binY[base + i];
+        if (!packed) sy += binY[base + i];
     }
     long long totw, toty;
     long long ow = block_excl_scan(sw, lscr, &totw);
-    long long oy = block_excl_scan(sy, lscr, &toty);
+    long long oy = packed ? 0 : block_excl_scan(sy, lscr, &toty);
     // cumulative sums in units of the total weight (FP64)
     double *Cw = (double *)binW;
     double *Cy = (double *)binY;
     const double fw = P.w_unit / P.W_tot, fy = P.y_unit / P.W_tot;
     for (int i = b0; i < b1; ++i) {
         ow += binW[base + i];
-        oy += binY[base + i];
-        Cw[base + i] = (double)ow * fw;
-        Cy[base + i] = (double)oy * fy;
+        if (packed) {
+            // the packed prefix holds (points << ybits) + flux sum exactly;
+            // points * the common weight is the unpacked weight prefix
+            const long long ys = packed_y(ow, P.ybits);
+            const long long cnt = (ow - ys) >> P.ybits;
+            Cw[base + i] = (double)(cnt * P.wq_one) * fw;
+            Cy[base + i] = (double)ys * fy;
+        } else {
+            oy += binY[base + i];
+            Cw[base + i] = (double)ow * fw;
+            Cy[base + i] = (double)oy * fy;
+        }
     }
     __syncthreads();
 
@@ -926,8 +964,9 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
             __syncthreads();
         } else {
             for (int i = 0; i < P.L; ++i) {
-                const long long wv = wp[(long long)i * NT];
+                long long wv = wp[(long long)i * NT];
                 const long long yv = yp[(long long)i * NT];
+                if (P.packed) wv = wv != 0 ? P.wq_one : 0;
                 if ((wv | yv) == 0) continue;
                 const double tv = tp[(long long)i * NT];
                 const int r = min(max(fold_bin(tv, period, invP, bd, inv_bd),
@@ -1129,7 +1168,12 @@ __global__ __launch_bounds__(NT) void k_main(Batch b, Out o, Work w,
 
         long long p = pbeg;
         while (p < pend) {
-            // greedy sub-batch of periods that fit in the LDS histogram
+            // greedy sub-batch of periods that fit in the LDS histogram.  A
+            // packed histogram takes 8 B per bin, so both halves of the LDS
+            // hold bins; each packed period also needs nb + 1 free slots
+            // above it for its FP64 flux prefix (finished last to first).
+            const bool packed = P.packed;
+            const int cap = packed ? 2 * lds_bins : lds_bins;
             double per[SUBMAX], inv[SUBMAX];
             int nb[SUBMAX], base[SUBMAX];
             int cnt = 0, used = 0;
@@ -1145,7 +1189,7 @@ __global__ __launch_bounds__(NT) void k_main(Batch b, Out o, Work w,
                 if (cnt == s && p + s < pend) {
                     const double ps = b.periods[p + s];
                     const int nbs = (int)ceil(ps / bd) + os;
-                    if (used + nbs + 1 <= lds_bins) {
+                    if (used + (packed ? 2 : 1) * (nbs + 1) <= cap) {
                         per[s] = ps;
                         inv[s] = 1.0 / ps;
                         nb[s] = nbs;
@@ -1170,7 +1214,7 @@ __global__ __launch_bounds__(NT) void k_main(Batch b, Out o, Work w,
             }
             for (int i = tid; i < used; i += NT) {
                 binW[i] = 0;
-                binY[i] = 0;
+                if (!packed) binY[i] = 0;
             }
             __syncthreads();
 
@@ -1183,37 +1227,49 @@ __global__ __launch_bounds__(NT) void k_main(Batch b, Out o, Work w,
                 const long long *wp = w.ws_w + P.ws_off + tid;
                 const long long *yp = w.ws_y + P.ws_off + tid;
                 const int L = P.L;
-#define ATG_BIN(C, E)                                                       \
+#define ATG_BIN(C, E, K)                                                    \
     case C:                                                                 \
-        bin_points<C, E>(tp, wp, yp, L, per, inv, nb, base, bd, inv_bd,     \
-                         P.tol_r, P.tol_x, binW, binY);                     \
+        bin_points<C, E, K>(tp, wp, yp, L, per, inv, nb, base, bd, inv_bd,  \
+                            P.tol_r, P.tol_x, binW, binY);                  \
         break;
+#define ATG_BIN8(E, K)                                                      \
+    switch (cnt) {                                                          \
+        ATG_BIN(1, E, K) ATG_BIN(2, E, K) ATG_BIN(3, E, K) ATG_BIN(4, E, K) \
+        ATG_BIN(5, E, K) ATG_BIN(6, E, K) ATG_BIN(7, E, K) ATG_BIN(8, E, K) \
+        default: break;                                                     \
+    }
                 if (b.mode == ATG_MODE_FAST) {
-                    switch (cnt) {
-                        ATG_BIN(1, false) ATG_BIN(2, false) ATG_BIN(3, false)
-                        ATG_BIN(4, false) ATG_BIN(5, false) ATG_BIN(6, false)
-                        ATG_BIN(7, false) ATG_BIN(8, false)
-                        default: break;
-                    }
+                    if (packed) ATG_BIN8(false, true)
+                    else ATG_BIN8(false, false)
                 } else {
-                    switch (cnt) {
-                        ATG_BIN(1, true) ATG_BIN(2, true) ATG_BIN(3, true)
-                        ATG_BIN(4, true) ATG_BIN(5, true) ATG_BIN(6, true)
-                        ATG_BIN(7, true) ATG_BIN(8, true)
-                        default: break;
-                    }
+                    if (packed) ATG_BIN8(true, true)
+                    else ATG_BIN8(true, false)
                 }
+#undef ATG_BIN8
 #undef ATG_BIN
             }
             __syncthreads();
 
-            int bse = 0;
-            for (int s = 0; s < cnt; ++s) {
-                const double ps = b.periods[p + s];
-                const int nbs = (int)ceil(ps / bd) + os;
-                finish_period(b, o, P, p + s, ps, nbs, bse, binW, binY, s_m,
-                              nd, lscr, dscr, iscr);
-                bse += nbs + 1;
+            if (packed) {
+                int end = used;
+                for (int s = cnt - 1; s >= 0; --s) {
+                    const double ps = b.periods[p + s];
+                    const int nbs = (int)ceil(ps / bd) + os;
+                    const int bse = end - (nbs + 1);
+                    finish_period(b, o, P, p + s, ps, nbs, bse, binW,
+                                  binW + (nbs + 1), s_m, nd, lscr, dscr,
+                                  iscr);
+                    end = bse;
+                }
+            } else {
+                int bse = 0;
+                for (int s = 0; s < cnt; ++s) {
+                    const double ps = b.periods[p + s];
+                    const int nbs = (int)ceil(ps / bd) + os;
+                    finish_period(b, o, P, p + s, ps, nbs, bse, binW, binY,
+                                  s_m, nd, lscr, dscr, iscr);
+                    bse += nbs + 1;
+                }
             }
             p += cnt;
         }
