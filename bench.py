@@ -363,7 +363,11 @@ def main():
                     "peak_GBps": PEAK_LDS / 1e9,
                     "frac": round(ach_lds / PEAK_LDS, 4),
                     "bytes_per_launch": per_launch_lds},
-            "hbm_algorithmic_bytes_per_launch": args.n_lc * 15000 * 24 +
+            # the survey's minimum (inputs 16 B/point, outputs) and what
+            # this design moves once: packed lane-major points 16 B/point,
+            # point-order copies for the phase tiles 24 B/point, outputs
+            "hbm_algorithmic_bytes_per_launch": hbm_in + hbm_out,
+            "hbm_design_bytes_per_launch": args.n_lc * 15000 * (16 + 24) +
             hbm_out,
         },
     }
@@ -394,12 +398,22 @@ def main():
                      all_depths=hp["depth"][a:z]), k=5)])
         host_ms = (time.perf_counter() - s) * 1e3
         hidx, hcnt = idx.cpu().numpy(), cnt.cpu().numpy()
-        same = sum(int(host_idx[i] == list(hidx[i, :int(hcnt[i])]))
-                   for i in range(args.n_lc))
+        from scipy.signal import find_peaks
+        same = tied = 0
+        for i in range(args.n_lc):
+            if host_idx[i] == list(hidx[i, :int(hcnt[i])]):
+                same += 1
+                continue
+            # numpy's default argsort orders equal peak powers differently
+            # from machine to machine; the device ranks the later first
+            x = hp["power"][po[i]:po[i + 1]]
+            pk = find_peaks(x)[0]
+            tied += int(len(np.unique(x[pk])) < len(pk))
         line["top_k"] = {"k": 5, "distance": 10,
                          "device_ms": round(dev_ms, 3),
                          "host_scipy_ms": round(host_ms, 1),
                          "curves_identical_to_host_rule": same,
+                         "other_curves_with_tied_peak_powers": tied,
                          "curves": args.n_lc}
         # plain form: one curve per call, host arrays in and out
         lat = []
