@@ -546,7 +546,12 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     // copies) reach at most 2n points, which takes cbits bits; the flux
     // field keeps 63 - cbits bits and its scale leaves 2 bits of headroom.
     const int cbits = ilogb(2.0 * n + 1.0) + 1;
-    P.packed = (iv_min == iv_max && 63 - cbits >= 40) ? 1 : 0;
+    // Fast mode only: the flux field keeps 63 - cbits bits (48 at 15k
+    // points, 44 at 200k), which moves power by up to ~1e-9 (15k) or
+    // ~1e-7 (200k) relative; parity mode keeps the 60-bit bins so that its
+    // error stays far below the near-tie window of the refinement.
+    P.packed = (b.mode == ATG_MODE_FAST && iv_min == iv_max &&
+                63 - cbits >= 40) ? 1 : 0;
     P.ybits = 63 - cbits;
     // scale so that the sum of |quantized| values stays below 2^60 (packed:
     // 2^(ybits-3)); the padding copies at most double a prefix sum.
@@ -1292,7 +1297,11 @@ __global__ __launch_bounds__(NT) void k_main(Batch b, Out o, Work w,
 
 namespace {
 
-constexpr int CAP = 16;            // refined periods per curve
+// Periods whose best windows hold the same points have equal int64 power,
+// so the near-tie set can be a long plateau of neighbouring periods (59 at
+// the 5,000-period benchmark grid); all of it must be recomputed, or an
+// unrefined member (exact integer value) outranks the refined ones.
+constexpr int CAP = 1024;          // refined periods per curve
 constexpr double TIE_REL = 1e-9;   // >> relative error of the int64 path
 
 __global__ __launch_bounds__(NT) void k_cands(Batch b, Out o, Work w) {

@@ -68,17 +68,42 @@ def measured_traffic(n_lc):
         return None
 
 
-def cpu_baseline(d, n_sample, objective):
+def cpu_baseline(d, n_sample, objective, gpu=None):
+    """The FP64 C oracle on the host cores over the first ``n_sample``
+    curves.  With ``gpu`` (host copies of the timed run's outputs and period
+    offsets) the same oracle outputs give the parity of those curves."""
     from oracle import oracle
     cores = min(len(os.sched_getaffinity(0)), 16)
+    refs = []
     s = time.perf_counter()
     for i in range(n_sample):
-        oracle.power(d["t"][i], d["y"][i], d["dy"][i], d["periods"],
-                     d["durations"], objective, n_threads=cores)
+        refs.append(oracle.power(d["t"][i], d["y"][i], d["dy"][i],
+                                 d["periods"], d["durations"], objective,
+                                 n_threads=cores))
     el = time.perf_counter() - s
-    return dict(value=n_sample / el, unit="LC/s", cores=cores, kind="port",
-                sample="%d of the 1,000 curves, full 5,000x5 grid, FP64 C "
-                "oracle (OpenMP over periods), %.1f s" % (n_sample, el))
+    out = dict(value=n_sample / el, unit="LC/s", cores=cores, kind="port",
+               sample="%d of the 1,000 curves, full 5,000x5 grid, FP64 C "
+               "oracle (OpenMP over periods), %.1f s" % (n_sample, el))
+    if gpu is None:
+        return out, None
+    h, po = gpu
+    rel_max = dt0_max = 0.0
+    same = exact = 0
+    for i, r in enumerate(refs):
+        g = {k: h[k][po[i]:po[i + 1]] for k in r}
+        fin = np.isfinite(r["power"]) & (r["power"] != 0)
+        rel = np.abs(g["power"][fin] - r["power"][fin]) / np.abs(
+            r["power"][fin])
+        rel_max = max(rel_max, float(rel.max()))
+        ib = int(np.argmax(r["power"]))
+        same += int(int(h["best_index"][i]) == ib)
+        dt0_max = max(dt0_max, abs(g["transit_time"][ib] -
+                                   r["transit_time"][ib]))
+        exact += int(all(g[k][ib] == r[k][ib] for k in r))
+    return out, {"curves": n_sample, "power_rel_err_max": float(
+        "%.3g" % rel_max), "best_period_same": "%d/%d" % (same, n_sample),
+        "best_fields_bit_exact": "%d/%d" % (exact, n_sample),
+        "t0_err_at_best_max_d": float("%.3g" % dt0_max)}
 
 
 def ragged_leg(ctx, n_lc, objective):
@@ -439,7 +464,10 @@ def main():
             line["served"] = None
         line["ragged"] = ragged_leg(ctx, args.ragged_lc, objective)
         line["served_stream"] = serve5_leg(args.serve5_seconds, objective)
-        line["cpu_baseline"] = cpu_baseline(d, args.cpu_sample, objective)
+        ctx.run(batch, objective=objective, mode="parity", out=out)
+        line["cpu_baseline"], line["parity_vs_oracle"] = cpu_baseline(
+            d, args.cpu_sample, objective,
+            gpu=(out.to_host(), batch.host_period_offsets))
     print(json.dumps(line), flush=True)
 
 

@@ -225,3 +225,27 @@ def test_long_baseline_parity(ctx, mode):
             ib = int(np.argmax(ref["power"]))
             for k in ref:
                 assert g[k][ib] == ref[k][ib], (i, k)
+
+
+def test_long_plateau_of_tied_periods(ctx):
+    """Curve 45 of the 1,000-curve workload (seed 2): 59 neighbouring
+    periods share the best window's points, so their integer powers are
+    equal; the oracle's FP64 rounding puts its maximum at index 4944, deep
+    inside the plateau.  Every plateau member must be recomputed in the
+    oracle's order for the best period and its fields to be the oracle's."""
+    from astro_transit_gpu_amd import DeviceBatch, synth
+    d = synth.config2(n_lc=46, seed=2)
+    t, y, dy = d["t"][45], d["y"][45], d["dy"][45]
+    ref = O.power(t, y, dy, d["periods"], d["durations"], "likelihood")
+    mx = ref["power"].max()
+    assert np.sum(ref["power"] >= mx * (1 - 1e-9)) > 16
+    print("plateau curve: oracle best", int(np.argmax(ref["power"])),
+          flush=True)
+    b = DeviceBatch.from_host([t], [y], [dy], d["periods"], d["durations"],
+                              device=ctx.device)
+    h = ctx.run(b, objective="likelihood").to_host()
+    assert_parity(h, ref, "plateau")
+    ib = int(np.argmax(ref["power"]))
+    assert int(h["best_index"][0]) == ib
+    for k in ref:
+        assert h[k][ib] == ref[k][ib], k
