@@ -546,12 +546,13 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     // copies) reach at most 2n points, which takes cbits bits; the flux
     // field keeps 63 - cbits bits and its scale leaves 2 bits of headroom.
     const int cbits = ilogb(2.0 * n + 1.0) + 1;
-    // Fast mode only: the flux field keeps 63 - cbits bits (48 at 15k
-    // points, 44 at 200k), which moves power by up to ~1e-9 (15k) or
-    // ~1e-7 (200k) relative; parity mode keeps the 60-bit bins so that its
-    // error stays far below the near-tie window of the refinement.
-    P.packed = (b.mode == ATG_MODE_FAST && iv_min == iv_max &&
-                63 - cbits >= 40) ? 1 : 0;
+    // The flux field keeps 63 - cbits bits (48 at 15k points, 44 at 200k),
+    // which moves power by up to ~1e-9 (15k) or ~1e-7 (200k) relative; the
+    // refinement widens its near-tie window by that bound (k_cands).  In
+    // parity mode packing is limited to n <= 65536 points, so that the
+    // error stays near 1e-9 and the window small.
+    P.packed = (iv_min == iv_max && 63 - cbits >= 40 &&
+                (b.mode == ATG_MODE_FAST || n <= 65536)) ? 1 : 0;
     P.ybits = 63 - cbits;
     // scale so that the sum of |quantized| values stays below 2^60 (packed:
     // 2^(ybits-3)); the padding copies at most double a prefix sum.
@@ -1314,10 +1315,33 @@ __global__ __launch_bounds__(NT) void k_cands(Batch b, Out o, Work w) {
     if (b.mode != ATG_MODE_PARITY || w.prm[lc].status != ATG_LC_OK ||
         w.lc_flags[lc])
         return;
+    const LcParams P = w.prm[lc];
+    // Relative error bound of a period's int64 power.  Unpacked bins: far
+    // below TIE_REL (0 here).  Packed bins (uniform weight w): every yq is
+    // within half a quantum q = y_unit of y*w, so the window statistic
+    // num = W_in*Y_tot/W_tot - Y_in is off by at most n_in*q with
+    // n_in = W_in/w, while num = depth*W_in*W_out/W_tot; the power (square
+    // of num for the likelihood, proportional to it for the snr) is then off
+    // by at most 2.5*q*W_tot/(w*|depth|*W_out) relative.  W_in follows from
+    // the period's own log likelihood 0.5*W_in*depth^2.
+    const double w1 = (double)P.wq_one * P.w_unit;
+    auto bound = [&](long long i) -> double {
+        if (!P.packed) return 0.0;
+        const double dep = o.depth[i];
+        if (!(fabs(dep) > 0.0)) return 0.0;
+        const double w_in = 2.0 * o.log_like[i] / (dep * dep);
+        const double w_out = fmax(P.W_tot - w_in, 1e-300);
+        return 2.5 * P.y_unit * P.W_tot / (w1 * fabs(dep) * w_out);
+    };
+    // lower bound of the exact maximum; a period is a candidate when its
+    // upper bound reaches it (TIE_REL covers the oracle's own FP64 rounding)
     double m = -INFINITY;
     for (long long i = threadIdx.x; i < np; i += NT) {
         const double v = o.power[po + i];
-        if (v > m) m = v;  // NaN never compares greater
+        if (v > -INFINITY) {  // NaN and -inf never count
+            const double lo = v - bound(po + i) * fabs(v);
+            if (lo > m) m = lo;
+        }
     }
     m = block_reduce(m, MaxOp(), dscr);
     if (!(m > -INFINITY)) return;
@@ -1325,7 +1349,11 @@ __global__ __launch_bounds__(NT) void k_cands(Batch b, Out o, Work w) {
     int count = 0;
     for (long long base = 0; base < np && count < CAP; base += NT) {
         const long long i = base + threadIdx.x;
-        const int f = (i < np && o.power[po + i] >= thr) ? 1 : 0;
+        int f = 0;
+        if (i < np) {
+            const double v = o.power[po + i];
+            f = (v > -INFINITY && v + bound(po + i) * fabs(v) >= thr) ? 1 : 0;
+        }
         int tot;
         const int e = block_excl_scan(f, iscr, &tot);
         if (f && count + e < CAP) s_c[count + e] = po + i;
@@ -1335,9 +1363,9 @@ __global__ __launch_bounds__(NT) void k_cands(Batch b, Out o, Work w) {
     __syncthreads();
     if (threadIdx.x == 0) s_base = atomicAdd(w.cand_count, (unsigned)count);
     __syncthreads();
-    if (threadIdx.x < count) {
-        w.cand_lc[s_base + threadIdx.x] = lc;
-        w.cand_p[s_base + threadIdx.x] = s_c[threadIdx.x];
+    for (int k = threadIdx.x; k < count; k += NT) {
+        w.cand_lc[s_base + k] = lc;
+        w.cand_p[s_base + k] = s_c[k];
     }
 }
 
