@@ -249,3 +249,33 @@ def test_long_plateau_of_tied_periods(ctx):
     assert int(h["best_index"][0]) == ib
     for k in ref:
         assert h[k][ib] == ref[k][ib], k
+
+
+def test_refinement_queue_beyond_one_workgroup(ctx):
+    """Flat flux: every window has depth 0, so all 1,500 periods tie at
+    power 0 and the refinement queues its full per-curve capacity (1,024,
+    more than one workgroup's threads).  The oracle's answer (first period,
+    all fields) must come back, and a second curve in the same batch must
+    be unaffected."""
+    from astro_transit_gpu_amd import DeviceBatch, synth
+    rng = np.random.default_rng(77)
+    t = np.sort(rng.uniform(0.0, 20.0, 800))
+    flat = np.ones_like(t)
+    y2 = synth.inject_box(t, 1.0 + 1e-3 * rng.standard_normal(t.size), 2.3,
+                          0.7, 0.1, 3e-3)
+    dy = np.full(t.size, 1e-3)
+    periods = np.geomspace(0.6, 9.0, 1500)
+    durs = np.array([0.05, 0.1])
+    print("flat curve: %d tied periods" % periods.size, flush=True)
+    b = DeviceBatch.from_host([t, t], [flat, y2], [dy, dy], periods, durs,
+                              device=ctx.device)
+    res = ctx.run(b, objective="likelihood")
+    h = res.to_host()
+    for i, y in enumerate((flat, y2)):
+        ref = O.power(t, y, dy, periods, durs, "likelihood")
+        g = res.curve(i, h)
+        assert_parity(g, ref, "curve %d" % i)
+        ib = int(np.argmax(ref["power"]))
+        assert int(h["best_index"][i]) == ib
+        for k in ref:
+            assert g[k][ib] == ref[k][ib], (i, k)
