@@ -105,7 +105,7 @@ struct Work {
     unsigned int *counter;
     double *ws_t;         // lane-major t'
     long long *ws_w;      // lane-major quantized ivar (packed curves: the
-                          // packed bin increment 1 << ybits + ws_y)
+                          // bin increment (1 << ybits) + ws_y, 0 for padding)
     long long *ws_y;      // lane-major quantized ivar * (y - median)
     double *nat_t;        // the same three arrays in point order, written
     long long *nat_w;     // only for time-sorted curves with periods whose
