@@ -1,23 +1,32 @@
 // atg_bls.hip -- Box Least Squares periodogram for MI355X (gfx950, wave64).
 //
-// Pipeline per call (all on one HIP stream):
-//   k_prep     one workgroup per light curve: t_ref = min(t), NaN / dy
-//              checks, grid checks, exact median of y (radix select on the
-//              order-preserving key of the FP64 value), fixed-point scales,
-//              then the "lane-major" point workspace: t' = t - t_ref (FP64),
-//              round(ivar * 2^sw) and round((y - med) * ivar * 2^sy) as int64.
-//   k_items    work-item offsets (ceil(periods / periods_per_item) per curve)
-//              and the reset of the persistent queue counter.
-//   k_main     persistent workgroups pull (curve, period tile) items from an
-//              atomic counter.  For up to 8 periods at once each lane walks
-//              its contiguous run of points, folds them in FP64 with the
-//              oracle's exact bin rule, merges consecutive points of the same
-//              bin in registers and adds each run to the LDS histogram with
-//              one 64-bit integer atomic (order-independent => bit-exact).
-//              Then per period: padding copy, int64 prefix scan, conversion
-//              to FP64, sliding-box scan over every duration, argmax with the
-//              oracle's first-wins tie order, and the 7 output fields.
+// Pipeline of atg_bls_run (all on the context's HIP stream):
+//   k_prep     one workgroup per light curve: t_ref = min(t), NaN / dy /
+//              grid checks, time order, exact median of y (radix select on
+//              the order-preserving key of the FP64 value), fixed-point
+//              scales, the lane-major point workspace (t' = t - t_ref in
+//              FP64, quantized weight and weight*flux as int64, or one packed
+//              int64 for uniform weights), point-order copies for the phase
+//              tiles, and in parity mode the oracle's FP64 totals.
+//   k_items    per-queue work-item offsets (curve c in queue c mod 8, one
+//              queue per XCD) and the queue resets.
+//   k_main     persistent workgroups pull (curve, period tile) items from
+//              their XCD's queue, then steal.  Up to 8 periods share a pass
+//              over the points: each lane folds its contiguous run of points
+//              in FP64 with the oracle's exact bin rule and adds each point
+//              to the LDS histogram with 64-bit integer atomics
+//              (order-independent, so bit-reproducible).  Then per period:
+//              padding copy, int64 prefix scan, conversion to FP64, sliding
+//              box scan over every duration, argmax with the oracle's
+//              first-wins order, the 7 output fields.  Periods whose bins
+//              exceed the histogram take phase tiles (big_period).
+//   k_cands    parity mode: the periods that can hold the oracle's maximum
+//              (near-ties within the int64 path's error bound) into a queue.
+//   k_exact    recomputes the queued periods in the oracle's operation order.
 //   k_final    per-curve argmax of power and status.
+// Further entry points: atg_bls_top_k (k_topk, peaks of each periodogram),
+// atg_bls_stats (k_stats, vetting statistics per candidate), atg_bls_model
+// (k_model, transit mask and box model).
 //
 // Semantics follow astropy BoxLeastSquares.power(method="fast") (the
 // reference's oracle, src/astrotransit_gpu/search/cpu_reference_bls.py:
