@@ -234,7 +234,8 @@ __device__ __forceinline__ int fold_exact(double t, double P, double bd) {
 // Cycle index floor(t/P) of the oracle's fold, with the same near-integer
 // fallback as fold_bin, as a double.
 __device__ __forceinline__ double fold_cycle(double t, double P,
-                                            double invP) {
+                                            double invP, bool fast = false) {
+    if (fast) return floor(t * invP);  // fast mode's fold, no fallback
     const double r = t * invP;
     double q = floor(r);
     const double fr = r - q;
@@ -246,7 +247,10 @@ __device__ __forceinline__ double fold_cycle(double t, double P,
 // Bin of t within cycle q (q = fold_cycle(t)), clamped to [0, nb]:
 // equal to min(max(fold_bin(t), 0), nb).
 __device__ __forceinline__ int fold_bin_in(double t, double P, double q,
-                                           double bd, double inv_bd, int nb) {
+                                           double bd, double inv_bd, int nb,
+                                           bool fast = false) {
+    if (fast)  // fast mode's fold: FMA phase, reciprocal only
+        return min(max((int)(fma(-P, q, t) * inv_bd) + 1, 0), nb);
     const double ph = t - P * q;
     const double x = ph * inv_bd;
     double xi = trunc(x);
@@ -259,16 +263,19 @@ __device__ __forceinline__ int fold_bin_in(double t, double P, double q,
 // For time-sorted points the key (cycle, bin) of the oracle's fold is
 // non-decreasing in the point index: floor(t/P) is monotone in t and, for a
 // fixed cycle, so are t - P*q, the division by bd and the truncation.  The
-// first index whose key is >= (c, s) is therefore a binary search.
+// first index whose key is >= (c, s) is therefore a binary search.  The
+// same holds for fast mode's fold (products and an FMA, each monotone).
 __device__ int lower_key(const double *tn, int n, double P, double invP,
-                         double bd, double inv_bd, int nb, double c, int s) {
+                         double bd, double inv_bd, int nb, double c, int s,
+                         bool fast = false) {
     int lo = 0, hi = n;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         const double t = tn[mid];
-        const double q = fold_cycle(t, P, invP);
+        const double q = fold_cycle(t, P, invP, fast);
         const bool less =
-            q < c || (q == c && fold_bin_in(t, P, q, bd, inv_bd, nb) < s);
+            q < c ||
+            (q == c && fold_bin_in(t, P, q, bd, inv_bd, nb, fast) < s);
         if (less) lo = mid + 1;
         else hi = mid;
     }
@@ -880,6 +887,9 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
     const double period = b.periods[p], invP = 1.0 / period;
     const double bd = P.bd, inv_bd = 1.0 / bd;
     const int nval = nb + 1;
+    // the bins of the mode's fold, so that results do not depend on which
+    // periods the tiling sends here
+    const bool fastm = b.mode == ATG_MODE_FAST;
     const bool like = b.objective == ATG_OBJ_LIKELIHOOD;
     const double Wt = P.W_tot, Yt = P.Y_tot;
     const double ytn = Yt / Wt, epsn = DBL_EPSILON / Wt;
@@ -888,7 +898,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
     const double *tn = w.nat_t + b.lc_off[P.lc_self];
     int ncyc = 0;
     if (P.natural)
-        ncyc = (int)fold_cycle(tn[P.n - 1], period, invP) + 1;
+        ncyc = (int)fold_cycle(tn[P.n - 1], period, invP, fastm) + 1;
     const bool ranged = P.natural && ncyc <= (1 << 20) &&
                         lds_bins - ncyc - mmax - 1 >= 64;
     const int S = ranged ? lds_bins - ncyc - mmax - 1 : lds_bins - mmax - 1;
@@ -929,7 +939,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
                 const int sb = ex ? ((e & 1) ? os + 1 : 1)
                                   : ((e & 1) ? hi + 1 : s0);
                 const int v = lower_key(tn, P.n, period, invP, bd, inv_bd,
-                                        nb, (double)c, sb);
+                                        nb, (double)c, sb, fastm);
                 (ex ? erng : rng)[2 * c + (e & 1)] = v;
             }
         }
@@ -943,7 +953,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
                     const long long wv = wn[j], yv = yn[j];
                     if ((wv | yv) == 0) continue;
                     const int r = fold_bin_in(tn[j], period, qc, bd, inv_bd,
-                                              nb);
+                                              nb, fastm);
                     if ((r < nb - os || r == nb) && r >= s0 && r <= hi) {
                         __hip_atomic_fetch_add(&binW[r - s0], wv,
                                                __ATOMIC_RELAXED,
@@ -963,7 +973,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
                         const long long wv = wn[j], yv = yn[j];
                         if ((wv | yv) == 0) continue;
                         const int r = fold_bin_in(tn[j], period, qc, bd,
-                                                  inv_bd, nb);
+                                                  inv_bd, nb, fastm);
                         const int d = r + nb - os - 1;
                         if (r >= 1 && r <= os && d >= s0 && d <= hi) {
                             __hip_atomic_fetch_add(
@@ -984,8 +994,11 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
                 if (P.packed) wv = wv != 0 ? P.wq_one : 0;
                 if ((wv | yv) == 0) continue;
                 const double tv = tp[(long long)i * NT];
-                const int r = min(max(fold_bin(tv, period, invP, bd, inv_bd),
-                                      0), nb);
+                const int r =
+                    fastm ? fold_bin_in(tv, period, floor(tv * invP), bd,
+                                        inv_bd, nb, true)
+                          : min(max(fold_bin(tv, period, invP, bd, inv_bd), 0),
+                                nb);
                 int j1, j2;
                 pad_targets(r, nb, os, j1, j2);
                 seg_add(j1, s0, hi, wv, yv, cw, cy, binW, binY);

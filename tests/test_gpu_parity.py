@@ -7,7 +7,7 @@ import pytest
 import torch
 
 from oracle import oracle as O
-from gpu_util import (GOLDEN, assert_parity, power_rel_err,
+from gpu_util import (GOLDEN, T0_ATOL, assert_parity, power_rel_err,
                       window_agreement)
 
 pytestmark = pytest.mark.gpu
@@ -156,6 +156,21 @@ def test_segmented_bins_bit_identical(name):
         assert b[k][ib] == c["out"][k][ib], k
 
 
+@pytest.mark.parametrize("name", ["cfg1_small_likelihood", "hetero_dy",
+                                  "unsorted_t"])
+def test_fast_mode_tiling_independent(name):
+    """Fast mode folds with its own rule in the phase tiles too, so its
+    (drifting) results do not depend on which periods the LDS budget sends
+    there."""
+    from astro_transit_gpu_amd import Context
+    c = O.load_golden([p for p in GOLDEN if p.endswith(name + ".npz")][0])
+    a = run_single(Context(0), c, mode="fast")
+    b = run_single(Context(0, tuning=dict(lds_bins=300)), c, mode="fast")
+    for k in ("power", "depth", "depth_err", "duration", "transit_time",
+              "depth_snr", "log_likelihood", "best_index"):
+        assert np.array_equal(a[k], b[k]), k
+
+
 @pytest.mark.parametrize("name", ["cfg1_likelihood", "cfg1_snr",
                                   "hetero_dy"])
 def test_fast_mode_drift(ctx, name):
@@ -202,7 +217,10 @@ def test_long_baseline_parity(ctx, mode):
     workload) on every 100th of its 50,000 periods 1-100 d: the long periods
     need up to 25,010 bins and take the phase-tiled path.  Parity mode holds
     the parity bar with every best-period field equal to the oracle's; fast
-    mode holds the same power and T0 bar."""
+    mode stays within its stated drift (points on bin edges of the 2-min grid
+    may change bin): its best period has the oracle's maximum power within
+    1e-6 and the oracle's T0 and duration there, fewer than 1% of periods
+    move beyond 1e-6 and none beyond 10%."""
     from astro_transit_gpu_amd import DeviceBatch, synth
     rng = np.random.default_rng(404)
     curves = [synth.config4_curve(rng) for _ in range(2)]
@@ -218,13 +236,22 @@ def test_long_baseline_parity(ctx, mode):
     for i, c in enumerate(curves):
         ref = O.power(c[0], c[1], c[2], periods, durs, "likelihood")
         g = res.curve(i, h)
-        rel = assert_parity(g, ref, "curve %d" % i)
-        print("%s curve %d rel %.2e agree %.4f" % (
-            mode, i, rel, window_agreement(g, ref)))
+        ib = int(np.argmax(ref["power"]))
         if mode == "parity":
-            ib = int(np.argmax(ref["power"]))
+            rel = assert_parity(g, ref, "curve %d" % i)
             for k in ref:
                 assert g[k][ib] == ref[k][ib], (i, k)
+        else:
+            r = np.abs(g["power"] / ref["power"] - 1.0)
+            rel = float(r.max())
+            assert np.mean(r > 1e-6) < 0.01 and rel < 0.1, (i, rel)
+            gb = int(h["best_index"][i])
+            assert ref["power"][gb] >= ref["power"][ib] * (1 - 1e-6), i
+            assert abs(g["transit_time"][ib] - ref["transit_time"][ib]) <= \
+                T0_ATOL, i
+            assert g["duration"][ib] == ref["duration"][ib], i
+        print("%s curve %d rel %.2e agree %.4f" % (
+            mode, i, rel, window_agreement(g, ref)))
 
 
 def test_long_plateau_of_tied_periods(ctx):
