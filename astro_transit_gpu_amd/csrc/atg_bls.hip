@@ -692,6 +692,15 @@ struct Score {
             idx = i2;
         }
     }
+    // for a candidate whose index exceeds every one offered before
+    __device__ __forceinline__ void offer_later(double q2, double d2,
+                                                int i2) {
+        if (q2 * den > q * d2) {
+            q = q2;
+            den = d2;
+            idx = i2;
+        }
+    }
     __device__ __forceinline__ void wave_reduce() {
         for (int o = 32; o > 0; o >>= 1) {
             const double oq = __shfl_xor(q, o), od = __shfl_xor(den, o);
@@ -777,20 +786,26 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
     const double ytn = Yt / Wt, epsn = DBL_EPSILON / Wt;
     const bool like = b.objective == ATG_OBJ_LIKELIHOOD;
     Score best;
-    for (int s = tid; s < nb; s += NT) {
-        const double w0 = Cw[base + s], y0 = Cy[base + s];
-        for (int k = 0; k < nd; ++k) {
-            const int m = s_m[k];
-            if (s + m > nb) continue;
-            const double win = Cw[base + s + m] - w0;
-            const double yin = Cy[base + s + m] - y0;
+    // duration-major: a thread meets its windows in increasing index
+    // (k * nval + s), so a later equal score never replaces the held one,
+    // and the four prefix values of a window are independent loads
+    for (int k = 0; k < nd; ++k) {
+        const int m = s_m[k];
+        const int send = min(nb - 1, nb - m);
+#pragma unroll 2
+        for (int s = tid; s <= send; s += NT) {
+            const double w0 = Cw[base + s], y0 = Cy[base + s];
+            const double we = Cw[base + s + m], ye = Cy[base + s + m];
+            const double win = we - w0;
+            const double yin = ye - y0;
             const double wout = 1.0 - win;
             if (win < epsn || wout < epsn) continue;
             // depth * W_in * W_out / W_tot^2 = y_tot*w_in - y_in
             const double num = fma(ytn, win, -yin);
             if (num < 0.0) continue;
-            best.offer(num * num, like ? win * wout * wout : win * wout,
-                       k * nval + s);
+            best.offer_later(num * num,
+                             like ? win * wout * wout : win * wout,
+                             k * nval + s);
         }
     }
     // workgroup argmax (highest score, then lowest (k, n))
