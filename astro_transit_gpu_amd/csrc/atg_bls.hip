@@ -1045,14 +1045,18 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
         }
         __syncthreads();
         const int nend = min(s0 + S, nb);
-        for (int n = s0 + tid; n < nend; n += NT) {
-            const int l = n - s0;
-            const double w0 = Cw[l], y0 = Cy[l];
-            for (int k = 0; k < nd; ++k) {
-                const int m = s_m[k];
-                if (n + m > nb) continue;
-                const double win = Cw[l + m] - w0;
-                const double yin = Cy[l + m] - y0;
+        // duration-major, as in finish_period (indices are not monotone
+        // across segments, so the full tie rule stays)
+        for (int k = 0; k < nd; ++k) {
+            const int m = s_m[k];
+            const int last = min(nend - 1, nb - m);
+#pragma unroll 2
+            for (int n = s0 + tid; n <= last; n += NT) {
+                const int l = n - s0;
+                const double w0 = Cw[l], y0 = Cy[l];
+                const double we = Cw[l + m], ye = Cy[l + m];
+                const double win = we - w0;
+                const double yin = ye - y0;
                 const double wout = 1.0 - win;
                 if (win < epsn || wout < epsn) continue;
                 const double num = fma(ytn, win, -yin);
