@@ -61,8 +61,14 @@ extern "C" {
 #define ATG_OBJ_SNR 1
 
 /* mode: reference power(method=...) */
-#define ATG_MODE_PARITY 0 /* int64 fixed-point bins: bit-reproducible      */
-#define ATG_MODE_FAST 1   /* FP32 bins with LDS float atomics              */
+/* PARITY: the oracle's exact bin rule, int64 fixed-point bins, near-tied
+ *   periods recomputed in the oracle's order: bit-reproducible, the best
+ *   period and its fields equal to the oracle's.
+ * FAST: fused-multiply-add fold without the bin-edge test, no
+ *   recomputation, int64 bins (packed for uniform weights): reproducible,
+ *   with a stated drift for points within ~1 ulp of a bin edge. */
+#define ATG_MODE_PARITY 0
+#define ATG_MODE_FAST 1
 
 typedef struct atg_ctx atg_ctx;
 
@@ -108,8 +114,8 @@ typedef struct atg_bls_out {
 /* Kernel tiling (chosen by the autotuner; 0 fields keep the current value).
  */
 typedef struct atg_tuning {
-    int32_t periods_per_item; /* periods per work item (1..8)              */
-    int32_t lds_bins;         /* bins per LDS pass (sets dynamic LDS size) */
+    int32_t periods_per_item; /* periods per work item (1..64)             */
+    int32_t lds_bins;         /* LDS histogram: lds_bins * 16 bytes        */
     int32_t wg_per_cu;        /* persistent workgroups per CU              */
     int32_t reserved;
 } atg_tuning;
