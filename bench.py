@@ -288,6 +288,66 @@ def precision_leg(n_lc, n_sample, objective, seconds_cap=None):
             "cpu_cores": cores, "generation_s": round(gen, 1)}
 
 
+def parity_sweep(n_lc, objective):
+    """Every curve of the first ``n_lc`` of the 1,000-curve workload and of
+    the ragged workload, parity and fast mode, against the FP64 oracle on the
+    host cores: worst relative power error, share of periods beyond 1e-6,
+    best-period agreement (a different best period whose oracle power equals
+    the maximum within 1e-12 is counted as a tie) and bit-exact best
+    fields."""
+    import torch
+    from oracle import oracle
+    from astro_transit_gpu_amd import Context, DeviceBatch, synth
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    ctx = Context(0)
+    out = {}
+    for name, d in (("1000x15k_5000p_5d", synth.config2(n_lc=n_lc, seed=2)),
+                    ("ragged", synth.config3(n_lc=n_lc, seed=3))):
+        per = d["periods"] if isinstance(d["periods"], list) else \
+            [d["periods"]] * n_lc
+        dur = d["durations"] if isinstance(d["durations"], list) else \
+            [d["durations"]] * n_lc
+        b = DeviceBatch.from_host(d["t"], d["y"], d["dy"], d["periods"],
+                                  d["durations"], device=ctx.device)
+        res = {m: ctx.run(b, objective=objective, mode=m).to_host()
+               for m in ("parity", "fast")}
+        po = b.host_period_offsets
+        st = {m: dict(rel=0.0, big=0, tot=0, same=0, tie=0, exact=0)
+              for m in res}
+        for i in range(n_lc):
+            r = oracle.power(d["t"][i], d["y"][i], d["dy"][i], per[i],
+                             dur[i], objective, n_threads=cores)
+            ib = int(np.argmax(r["power"]))
+            fin = np.isfinite(r["power"]) & (r["power"] != 0)
+            for m, h in res.items():
+                g = {k: h[k][po[i]:po[i + 1]] for k in r}
+                rel = np.abs(g["power"][fin] / r["power"][fin] - 1.0)
+                s_ = st[m]
+                s_["rel"] = max(s_["rel"], float(rel.max()))
+                s_["big"] += int((rel > 1e-6).sum())
+                s_["tot"] += int(rel.size)
+                gb = int(h["best_index"][i])
+                s_["same"] += int(gb == ib)
+                s_["tie"] += int(gb != ib and r["power"][gb] >=
+                                 r["power"][ib] * (1 - 1e-12))
+                s_["exact"] += int(all(g[k][ib] == r[k][ib] for k in r))
+            if i % 100 == 99:
+                print("parity sweep: %s curve %d" % (name, i),
+                      file=sys.stderr, flush=True)
+        out[name] = {m: {"power_rel_err_max": float("%.3g" % v["rel"]),
+                         "frac_periods_rel_gt_1e-6":
+                             float("%.3g" % (v["big"] / max(v["tot"], 1))),
+                         "best_period_same": "%d/%d" % (v["same"], n_lc),
+                         "other_best_with_equal_power": v["tie"],
+                         "best_fields_bit_exact": "%d/%d" % (v["exact"],
+                                                             n_lc)}
+                     for m, v in st.items()}
+        del b, res
+        torch.cuda.empty_cache()
+    return {"metric": "parity sweep against the FP64 oracle", "curves": n_lc,
+            "objective": objective, "cpu_cores": cores, "results": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -306,6 +366,9 @@ def main():
                     help="keep the library's default tiling")
     ap.add_argument("--precision", action="store_true",
                     help="run only the long-baseline precision study")
+    ap.add_argument("--parity-sweep", type=int, default=0, metavar="N",
+                    help="run only the oracle comparison of N curves of the "
+                    "1,000-curve and ragged workloads, parity and fast mode")
     ap.add_argument("--precision-lc", type=int, default=512)
     ap.add_argument("--precision-sample", type=int, default=4)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -318,6 +381,10 @@ def main():
               "notes).", file=sys.stderr)
         sys.exit(2)
 
+    if args.parity_sweep:
+        print(json.dumps(parity_sweep(args.parity_sweep, "likelihood")),
+              flush=True)
+        return
     if args.precision:
         print(json.dumps(precision_leg(args.precision_lc,
                                        args.precision_sample, "likelihood")),
