@@ -38,6 +38,7 @@
 // is written.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -1345,6 +1346,11 @@ namespace {
 // unrefined member (exact integer value) outranks the refined ones.
 constexpr int CAP = 1024;          // refined periods per curve
 constexpr double TIE_REL = 1e-9;   // >> relative error of the int64 path
+// k_exact's FP64 histogram: one wave per queued period walks its points
+// sequentially, so the kernel is latency-bound; a small LDS histogram (16 KB)
+// lets 8 such waves share a CU (longer periods take the ranged segments).
+constexpr int EXACT_BINS = 1024;
+constexpr int EXACT_WG_PER_CU = 8;
 
 __global__ __launch_bounds__(NT) void k_cands(Batch b, Out o, Work w) {
     __shared__ double dscr[NW];
@@ -2551,10 +2557,6 @@ int atg_bls_run(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o) {
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds);
         if (e != hipSuccess) return hip_fail(c, e, "hipFuncSetAttribute");
-        e = hipFuncSetAttribute((const void *)k_exact,
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
-        if (e != hipSuccess) return hip_fail(c, e, "hipFuncSetAttribute");
         c->lds_attr_set = lds;
     }
     hipLaunchKernelGGL(k_prep, dim3(nlc), dim3(NT), 0, c->stream, kb, w,
@@ -2584,8 +2586,9 @@ int atg_bls_run(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o) {
         c->pending.emplace_back(ev0, ev1);
     }
     hipLaunchKernelGGL(k_cands, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w);
-    hipLaunchKernelGGL(k_exact, dim3(c->num_cu * 2), dim3(64), lds, c->stream,
-                       kb, ko, w, c->tune.lds_bins);
+    const int ex_bins = std::min(c->tune.lds_bins, EXACT_BINS);
+    hipLaunchKernelGGL(k_exact, dim3(c->num_cu * EXACT_WG_PER_CU), dim3(64),
+                       (size_t)ex_bins * 16, c->stream, kb, ko, w, ex_bins);
     hipLaunchKernelGGL(k_final, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(c, e, "kernel launch");
