@@ -321,12 +321,13 @@ __device__ __forceinline__ void bin_points(
                 const double fr = r - q;
                 const double ph = tv - per[s] * q;
                 const double x = ph * inv_bd;
-                const double xi = trunc(x);
-                const double fx = x - xi;
+                // x - floor(x) in one instruction; for an unflagged x >= 0
+                // the conversion's truncation is that floor
+                const double fx = __builtin_amdgcn_fract(x);
                 fl[s] = (fr <= tol_r) | (fr >= hr) | (fx <= tol_x) |
                         (fx >= hx);
                 any |= fl[s];
-                at[s] = base[s] + 1 + (int)xi;
+                at[s] = base[s] + 1 + (int)x;
             } else {
                 // fast mode: FMA phase and rounded reciprocals only; a point
                 // within ~1e-15 of a bin edge may land in the neighbour bin
