@@ -6,9 +6,10 @@ backed by hand-written gfx950 HIP kernels behind a C ABI
 (include/atg_bls.h, libatg_bls.so).
 """
 from .api import BLSResult, BoxLeastSquaresGPU, Candidate, get_top_k_candidates
-from .engine import Context, DeviceBatch, DeviceResult, default_context
+from .engine import (Context, DeviceBatch, DeviceBest, DeviceResult,
+                     default_context)
 
 __version__ = "0.1.0"
 __all__ = ["BoxLeastSquaresGPU", "BLSResult", "Candidate",
-           "get_top_k_candidates", "Context", "DeviceBatch", "DeviceResult",
-           "default_context"]
+           "get_top_k_candidates", "Context", "DeviceBatch", "DeviceBest",
+           "DeviceResult", "default_context"]

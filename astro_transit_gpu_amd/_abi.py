@@ -36,7 +36,8 @@ EXPORTS = (
     "atg_ctx_set_stream", "atg_last_error", "atg_ctx_set_tuning",
     "atg_ctx_get_tuning", "atg_bls_workspace_size", "atg_bls_run",
     "atg_bls_run_sync", "atg_timing_main_kernel", "atg_bls_top_k",
-    "atg_bls_stats", "atg_bls_model",
+    "atg_bls_stats", "atg_bls_model", "atg_bls_run_best",
+    "atg_bls_workspace_size_best",
 )
 
 _vp = ctypes.c_void_p
@@ -60,6 +61,15 @@ class Out(ctypes.Structure):
         ("power", _vp), ("depth", _vp), ("depth_err", _vp),
         ("duration", _vp), ("transit_time", _vp), ("depth_snr", _vp),
         ("log_likelihood", _vp), ("best_index", _vp), ("status", _vp),
+    ]
+
+
+class Best(ctypes.Structure):
+    _fields_ = [
+        ("power", _vp), ("depth", _vp), ("depth_err", _vp),
+        ("duration", _vp), ("transit_time", _vp), ("depth_snr", _vp),
+        ("log_likelihood", _vp), ("period", _vp), ("best_index", _vp),
+        ("status", _vp),
     ]
 
 
@@ -104,6 +114,12 @@ def lib():
     for fn in (L.atg_bls_run, L.atg_bls_run_sync):
         fn.restype = ctypes.c_int
         fn.argtypes = [_vp, ctypes.POINTER(Batch), ctypes.POINTER(Out)]
+    L.atg_bls_run_best.restype = ctypes.c_int
+    L.atg_bls_run_best.argtypes = [_vp, ctypes.POINTER(Batch),
+                                   ctypes.POINTER(Best)]
+    L.atg_bls_workspace_size_best.restype = ctypes.c_int
+    L.atg_bls_workspace_size_best.argtypes = [
+        _vp, ctypes.POINTER(Batch), ctypes.POINTER(ctypes.c_uint64)]
     L.atg_bls_top_k.restype = ctypes.c_int
     L.atg_bls_top_k.argtypes = [_vp, ctypes.POINTER(Batch),
                                 ctypes.POINTER(Out), ctypes.c_int,

@@ -23,7 +23,10 @@
 //   k_cands    parity mode: the periods that can hold the oracle's maximum
 //              (near-ties within the int64 path's error bound) into a queue.
 //   k_exact    recomputes the queued periods in the oracle's operation order.
-//   k_final    per-curve argmax of power and status.
+//   k_final    per-curve argmax of power and status (and, for a best-only
+//              run, the winner's fields).
+// atg_bls_run_best runs the same sequence with the per-period arrays in the
+// workspace and returns only the winners.
 // Further entry points: atg_bls_top_k (k_topk, peaks of each periodogram),
 // atg_bls_stats (k_stats, vetting statistics per candidate), atg_bls_model
 // (k_model, transit mask and box model).
@@ -2224,7 +2227,17 @@ __global__ __launch_bounds__(NT) void k_model(long long n, const double *t,
 
 // ----------------------------------------------------------------- k_final
 
-__global__ __launch_bounds__(NT) void k_final(Batch b, Out o, Work w) {
+// Winner records of a best-only run (per curve); power == nullptr in a
+// full run.
+struct Winners {
+    double *power, *depth, *depth_err, *duration, *transit_time, *depth_snr,
+        *log_like, *period;
+    long long *best_index;
+    int *status;
+};
+
+__global__ __launch_bounds__(NT) void k_final(Batch b, Out o, Work w,
+                                              Winners bo) {
     __shared__ double dscr[NW];
     __shared__ long long lscr[NW];
     __shared__ int iscr[NW];
@@ -2276,11 +2289,25 @@ __global__ __launch_bounds__(NT) void k_final(Batch b, Out o, Work w) {
                 bn = iscr[i];
             }
         }
-        if (o.best_index) o.best_index[lc] = (bi == LLONG_MAX) ? 0 : bi;
-        if (o.status) {
-            int st = w.prm[lc].status;
-            if (st == ATG_LC_OK && w.lc_flags[lc]) st = ATG_LC_TOO_MANY_BINS;
-            o.status[lc] = st;
+        // bi stays LLONG_MAX only for a curve without periods
+        const bool have = bi != LLONG_MAX;
+        int st = w.prm[lc].status;
+        if (st == ATG_LC_OK && w.lc_flags[lc]) st = ATG_LC_TOO_MANY_BINS;
+        if (o.best_index) o.best_index[lc] = have ? bi : 0;
+        if (o.status) o.status[lc] = st;
+        if (bo.power) {  // best-only epilogue: gather the winner's fields
+            const double nan = __longlong_as_double(0x7ff8000000000000ll);
+            const long long q = po + (have ? bi : 0);
+            bo.power[lc] = have ? o.power[q] : nan;
+            bo.depth[lc] = have ? o.depth[q] : nan;
+            bo.depth_err[lc] = have ? o.depth_err[q] : nan;
+            bo.duration[lc] = have ? o.duration[q] : nan;
+            bo.transit_time[lc] = have ? o.transit_time[q] : nan;
+            bo.depth_snr[lc] = have ? o.depth_snr[q] : nan;
+            bo.log_like[lc] = have ? o.log_like[q] : nan;
+            bo.period[lc] = have ? b.periods[q] : nan;
+            if (bo.best_index) bo.best_index[lc] = have ? bi : 0;
+            if (bo.status) bo.status[lc] = st;
         }
         w.lc_flags[lc] = 0;
     }
@@ -2353,8 +2380,8 @@ Layout layout_of(const atg_bls_batch *b) {
     return L;
 }
 
-int check_batch(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o) {
-    if (!b || !o) return fail(c, ATG_ERR_ARG, "null batch or output");
+int check_batch(atg_ctx *c, const atg_bls_batch *b) {
+    if (!b) return fail(c, ATG_ERR_ARG, "null batch");
     if (b->n_lc < 1 || b->n_lc > INT_MAX)
         return fail(c, ATG_ERR_ARG, "n_lc must be in [1, 2^31)");
     if (b->n_points < 0 || b->n_periods < 1 || b->n_durations < 1)
@@ -2375,10 +2402,13 @@ int check_batch(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o) {
         return fail(c, ATG_ERR_ARG, "objective must be likelihood or snr");
     if (b->mode != ATG_MODE_PARITY && b->mode != ATG_MODE_FAST)
         return fail(c, ATG_ERR_ARG, "mode must be parity or fast");
-    if (!o->power || !o->depth || !o->depth_err || !o->duration ||
-        !o->transit_time || !o->depth_snr || !o->log_likelihood)
-        return fail(c, ATG_ERR_ARG, "null output array");
     return ATG_OK;
+}
+
+// The 7 per-period arrays of a best-only run, in the workspace after the
+// layout's own regions.
+size_t best_bytes(const atg_bls_batch *b) {
+    return align256(7 * (size_t)b->n_periods * sizeof(double));
 }
 
 int ensure_workspace(atg_ctx *c, size_t need) {
@@ -2425,6 +2455,117 @@ Batch to_batch(const atg_bls_batch *b) {
     k.objective = b->objective;
     k.mode = b->mode;
     return k;
+}
+
+// The launch sequence of atg_bls_run (bst == nullptr: o holds the caller's
+// per-period arrays) and atg_bls_run_best (o == nullptr: the per-period
+// arrays live in the workspace, k_final gathers the winners into bst).
+int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
+             const atg_bls_best *bst) {
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
+    const Layout L = layout_of(b);
+    int rc = ensure_workspace(c, L.total + (bst ? best_bytes(b) : 0));
+    if (rc != ATG_OK) return rc;
+    char *base = (char *)c->ws;
+    Work w;
+    w.prm = (LcParams *)(base + L.prm);
+    w.lc_flags = (int *)(base + L.flags);
+    w.item_off = (long long *)(base + L.items);
+    w.counter = (unsigned int *)(base + L.counter);
+    w.qtot = (long long *)(base + L.queues);
+    w.qnext = (unsigned int *)(base + L.queues + NQ * sizeof(long long));
+    w.ws_t = (double *)(base + L.t);
+    w.ws_w = (long long *)(base + L.w);
+    w.ws_y = (long long *)(base + L.y);
+    w.nat_t = (double *)(base + L.nt);
+    w.nat_w = (long long *)(base + L.nw);
+    w.nat_y = (long long *)(base + L.ny);
+    w.cand_count = w.counter + 1;
+    w.cand_next = w.counter + 2;
+    w.cand_lc = (int *)(base + L.cand_lc);
+    w.cand_p = (long long *)(base + L.cand_p);
+    Batch kb = to_batch(b);
+    Out ko;
+    Winners kbest{};
+    if (bst) {
+        double *f = (double *)(base + L.total);
+        const size_t np = (size_t)b->n_periods;
+        ko.power = f;
+        ko.depth = f + np;
+        ko.depth_err = f + 2 * np;
+        ko.duration = f + 3 * np;
+        ko.transit_time = f + 4 * np;
+        ko.depth_snr = f + 5 * np;
+        ko.log_like = f + 6 * np;
+        ko.best_index = nullptr;
+        ko.status = nullptr;
+        kbest.power = bst->power;
+        kbest.depth = bst->depth;
+        kbest.depth_err = bst->depth_err;
+        kbest.duration = bst->duration;
+        kbest.transit_time = bst->transit_time;
+        kbest.depth_snr = bst->depth_snr;
+        kbest.log_like = bst->log_likelihood;
+        kbest.period = bst->period;
+        kbest.best_index = (long long *)bst->best_index;
+        kbest.status = bst->status;
+    } else {
+        ko.power = o->power;
+        ko.depth = o->depth;
+        ko.depth_err = o->depth_err;
+        ko.duration = o->duration;
+        ko.transit_time = o->transit_time;
+        ko.depth_snr = o->depth_snr;
+        ko.log_like = o->log_likelihood;
+        ko.best_index = (long long *)o->best_index;
+        ko.status = o->status;
+    }
+
+    const int nlc = (int)b->n_lc;
+    const size_t lds = (size_t)c->tune.lds_bins * 16;
+    if (lds > c->lds_attr_set) {
+        e = hipFuncSetAttribute((const void *)k_main,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+        if (e != hipSuccess) return hip_fail(c, e, "hipFuncSetAttribute");
+        c->lds_attr_set = lds;
+    }
+    hipLaunchKernelGGL(k_prep, dim3(nlc), dim3(NT), 0, c->stream, kb, w,
+                       c->tune.lds_bins);
+    hipLaunchKernelGGL(k_items, dim3(1), dim3(NT), 0, c->stream,
+                       (long long)b->n_lc, kb.per_off,
+                       c->tune.periods_per_item, w);
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (c->timing) {
+        for (int i = 0; i < 2; ++i) {
+            hipEvent_t ev;
+            if (!c->spare.empty()) {
+                ev = c->spare.back();
+                c->spare.pop_back();
+            } else if (hipEventCreate(&ev) != hipSuccess) {
+                return fail(c, ATG_ERR_HIP, "hipEventCreate");
+            }
+            (i ? ev1 : ev0) = ev;
+        }
+        hipEventRecord(ev0, c->stream);
+    }
+    const int grid = c->num_cu * c->tune.wg_per_cu;
+    hipLaunchKernelGGL(k_main, dim3(grid), dim3(NT), lds, c->stream, kb, ko,
+                       w, c->tune.periods_per_item, c->tune.lds_bins);
+    if (c->timing) {
+        hipEventRecord(ev1, c->stream);
+        c->pending.emplace_back(ev0, ev1);
+    }
+    hipLaunchKernelGGL(k_cands, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w);
+    const int ex_bins = std::min(c->tune.lds_bins, EXACT_BINS);
+    hipLaunchKernelGGL(k_exact, dim3(c->num_cu * EXACT_WG_PER_CU), dim3(64),
+                       (size_t)ex_bins * 16, c->stream, kb, ko, w, ex_bins);
+    hipLaunchKernelGGL(k_final, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w,
+                       kbest);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(c, e, "kernel launch");
+    return ATG_OK;
 }
 
 }  // namespace
@@ -2514,85 +2655,32 @@ int atg_bls_workspace_size(const atg_ctx *c, const atg_bls_batch *b,
 
 int atg_bls_run(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o) {
     if (!c) return ATG_ERR_ARG;
-    int rc = check_batch(c, b, o);
+    int rc = check_batch(c, b);
     if (rc != ATG_OK) return rc;
-    hipError_t e = hipSetDevice(c->device);
-    if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
-    const Layout L = layout_of(b);
-    rc = ensure_workspace(c, L.total);
-    if (rc != ATG_OK) return rc;
-    char *base = (char *)c->ws;
-    Work w;
-    w.prm = (LcParams *)(base + L.prm);
-    w.lc_flags = (int *)(base + L.flags);
-    w.item_off = (long long *)(base + L.items);
-    w.counter = (unsigned int *)(base + L.counter);
-    w.qtot = (long long *)(base + L.queues);
-    w.qnext = (unsigned int *)(base + L.queues + NQ * sizeof(long long));
-    w.ws_t = (double *)(base + L.t);
-    w.ws_w = (long long *)(base + L.w);
-    w.ws_y = (long long *)(base + L.y);
-    w.nat_t = (double *)(base + L.nt);
-    w.nat_w = (long long *)(base + L.nw);
-    w.nat_y = (long long *)(base + L.ny);
-    w.cand_count = w.counter + 1;
-    w.cand_next = w.counter + 2;
-    w.cand_lc = (int *)(base + L.cand_lc);
-    w.cand_p = (long long *)(base + L.cand_p);
-    Batch kb = to_batch(b);
-    Out ko;
-    ko.power = o->power;
-    ko.depth = o->depth;
-    ko.depth_err = o->depth_err;
-    ko.duration = o->duration;
-    ko.transit_time = o->transit_time;
-    ko.depth_snr = o->depth_snr;
-    ko.log_like = o->log_likelihood;
-    ko.best_index = (long long *)o->best_index;
-    ko.status = o->status;
+    if (!o || !o->power || !o->depth || !o->depth_err || !o->duration ||
+        !o->transit_time || !o->depth_snr || !o->log_likelihood)
+        return fail(c, ATG_ERR_ARG, "null output array");
+    return run_impl(c, b, o, nullptr);
+}
 
-    const int nlc = (int)b->n_lc;
-    const size_t lds = (size_t)c->tune.lds_bins * 16;
-    if (lds > c->lds_attr_set) {
-        e = hipFuncSetAttribute((const void *)k_main,
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
-        if (e != hipSuccess) return hip_fail(c, e, "hipFuncSetAttribute");
-        c->lds_attr_set = lds;
-    }
-    hipLaunchKernelGGL(k_prep, dim3(nlc), dim3(NT), 0, c->stream, kb, w,
-                       c->tune.lds_bins);
-    hipLaunchKernelGGL(k_items, dim3(1), dim3(NT), 0, c->stream,
-                       (long long)b->n_lc, kb.per_off,
-                       c->tune.periods_per_item, w);
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (c->timing) {
-        for (int i = 0; i < 2; ++i) {
-            hipEvent_t ev;
-            if (!c->spare.empty()) {
-                ev = c->spare.back();
-                c->spare.pop_back();
-            } else if (hipEventCreate(&ev) != hipSuccess) {
-                return fail(c, ATG_ERR_HIP, "hipEventCreate");
-            }
-            (i ? ev1 : ev0) = ev;
-        }
-        hipEventRecord(ev0, c->stream);
-    }
-    const int grid = c->num_cu * c->tune.wg_per_cu;
-    hipLaunchKernelGGL(k_main, dim3(grid), dim3(NT), lds, c->stream, kb, ko,
-                       w, c->tune.periods_per_item, c->tune.lds_bins);
-    if (c->timing) {
-        hipEventRecord(ev1, c->stream);
-        c->pending.emplace_back(ev0, ev1);
-    }
-    hipLaunchKernelGGL(k_cands, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w);
-    const int ex_bins = std::min(c->tune.lds_bins, EXACT_BINS);
-    hipLaunchKernelGGL(k_exact, dim3(c->num_cu * EXACT_WG_PER_CU), dim3(64),
-                       (size_t)ex_bins * 16, c->stream, kb, ko, w, ex_bins);
-    hipLaunchKernelGGL(k_final, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w);
-    e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(c, e, "kernel launch");
+int atg_bls_run_best(atg_ctx *c, const atg_bls_batch *b,
+                     const atg_bls_best *best) {
+    if (!c) return ATG_ERR_ARG;
+    int rc = check_batch(c, b);
+    if (rc != ATG_OK) return rc;
+    if (!best || !best->power || !best->depth || !best->depth_err ||
+        !best->duration || !best->transit_time || !best->depth_snr ||
+        !best->log_likelihood || !best->period)
+        return fail(c, ATG_ERR_ARG, "null output array");
+    return run_impl(c, b, nullptr, best);
+}
+
+int atg_bls_workspace_size_best(const atg_ctx *c, const atg_bls_batch *b,
+                                uint64_t *bytes) {
+    (void)c;
+    if (!b || !bytes || b->n_lc < 1 || b->n_points < 0 || b->n_periods < 0)
+        return ATG_ERR_ARG;
+    *bytes = layout_of(b).total + best_bytes(b);
     return ATG_OK;
 }
 

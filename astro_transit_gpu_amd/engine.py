@@ -157,6 +157,41 @@ class DeviceResult:
         return {k: h[k][a:b] for k in FIELDS}
 
 
+BEST_FIELDS = FIELDS + ("period",)
+
+
+@dataclass
+class DeviceBest:
+    """Per-curve winner of a best-only run: the 7 fields of the best period,
+    the period itself, its index within the curve's grid, the status."""
+    power: torch.Tensor
+    depth: torch.Tensor
+    depth_err: torch.Tensor
+    duration: torch.Tensor
+    transit_time: torch.Tensor
+    depth_snr: torch.Tensor
+    log_likelihood: torch.Tensor
+    period: torch.Tensor
+    best_index: torch.Tensor
+    status: torch.Tensor
+
+    @staticmethod
+    def empty(n_lc, device):
+        f = [torch.empty(n_lc, dtype=torch.float64, device=device)
+             for _ in BEST_FIELDS]
+        return DeviceBest(*f,
+                          best_index=torch.empty(n_lc, dtype=torch.int64,
+                                                 device=device),
+                          status=torch.empty(n_lc, dtype=torch.int32,
+                                             device=device))
+
+    def to_host(self):
+        out = {k: getattr(self, k).cpu().numpy() for k in BEST_FIELDS}
+        out["best_index"] = self.best_index.cpu().numpy()
+        out["status"] = self.status.cpu().numpy()
+        return out
+
+
 class Context:
     """A libatg_bls context bound to one GPU and one stream."""
 
@@ -220,6 +255,12 @@ class Context:
         return n, ms.value
 
     def _structs(self, batch, out, objective, mode, oversample):
+        b = self._batch_struct(batch, objective, mode, oversample)
+        o = _abi.Out(*[_ptr(getattr(out, k)) for k in FIELDS],
+                     best_index=_ptr(out.best_index), status=_ptr(out.status))
+        return b, o
+
+    def _batch_struct(self, batch, objective, mode, oversample):
         if objective not in _abi.OBJECTIVES:
             raise ValueError("objective must be 'likelihood' or 'snr'")
         if mode not in _abi.MODES:
@@ -250,9 +291,7 @@ class Context:
             oversample=int(oversample),
             objective=_abi.OBJECTIVES[objective], mode=_abi.MODES[mode],
             reserved=0)
-        o = _abi.Out(*[_ptr(getattr(out, k)) for k in FIELDS],
-                     best_index=_ptr(out.best_index), status=_ptr(out.status))
-        return b, o
+        return b
 
     def run(self, batch: DeviceBatch, objective="likelihood", mode="parity",
             oversample=10, out: Optional[DeviceResult] = None, sync=True):
@@ -264,6 +303,23 @@ class Context:
         b, o = self._structs(batch, out, objective, mode, oversample)
         fn = self._lib.atg_bls_run_sync if sync else self._lib.atg_bls_run
         self._check(fn(self._h, ctypes.byref(b), ctypes.byref(o)))
+        return out
+
+    def run_best(self, batch: DeviceBatch, objective="likelihood",
+                 mode="parity", oversample=10, out: Optional[DeviceBest] = None,
+                 sync=True):
+        """Best-only run (atg_bls_run_best): the winner of every curve,
+        gathered on the device; the per-period arrays stay in the library's
+        workspace.  Every field equals the full run's at its best_index."""
+        if out is None:
+            out = DeviceBest.empty(batch.n_lc, self.device)
+        b = self._batch_struct(batch, objective, mode, oversample)
+        o = _abi.Best(*[_ptr(getattr(out, k)) for k in BEST_FIELDS],
+                      best_index=_ptr(out.best_index), status=_ptr(out.status))
+        self._check(self._lib.atg_bls_run_best(self._h, ctypes.byref(b),
+                                               ctypes.byref(o)))
+        if sync:
+            self.stream.synchronize()
         return out
 
     def capture(self, batch: DeviceBatch, out: DeviceResult,
@@ -407,6 +463,14 @@ class Context:
         n = ctypes.c_uint64(0)
         self._check(self._lib.atg_bls_workspace_size(self._h, ctypes.byref(b),
                                                      ctypes.byref(n)))
+        return int(n.value)
+
+    def workspace_bytes_best(self, batch: DeviceBatch):
+        """Workspace of a best-only run of ``batch``."""
+        b = self._batch_struct(batch, "likelihood", "parity", 10)
+        n = ctypes.c_uint64(0)
+        self._check(self._lib.atg_bls_workspace_size_best(
+            self._h, ctypes.byref(b), ctypes.byref(n)))
         return int(n.value)
 
 

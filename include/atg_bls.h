@@ -163,6 +163,38 @@ int atg_bls_run(atg_ctx *ctx, const atg_bls_batch *batch,
 int atg_bls_run_sync(atg_ctx *ctx, const atg_bls_batch *batch,
                      const atg_bls_out *out);
 
+/* Caller-allocated device outputs of a best-only run: one entry per light
+ * curve, the fields of its best period (the period at the index
+ * atg_bls_out.best_index would hold; NaN fields for a curve without
+ * periods). */
+typedef struct atg_bls_best {
+    double *power;           /* [n_lc]                                     */
+    double *depth;
+    double *depth_err;
+    double *duration;
+    double *transit_time;    /* absolute: phase + t_ref                    */
+    double *depth_snr;
+    double *log_likelihood;
+    double *period;          /* [n_lc] the best period itself              */
+    int64_t *best_index;     /* [n_lc] within the curve's grid; may be NULL */
+    int32_t *status;         /* [n_lc] ATG_LC_*; may be NULL               */
+} atg_bls_best;
+
+/* Best-only screening (asynchronous on the context stream).  Replaces
+ * run_vbls_massive (search/vbls.py:62-188, kernels loaded at vbls.py:12-55
+ * from search/kernels/vbls_blackwell_v39.cu or vbls.cu), which keeps
+ * only each target's winner: the same launch sequence as atg_bls_run, with
+ * the per-period arrays in the library's workspace (56 more bytes per
+ * period) and the winners' fields gathered on the device, so that only
+ * n_lc records leave it.  Every field equals the one atg_bls_run writes at
+ * out->best_index. */
+int atg_bls_run_best(atg_ctx *ctx, const atg_bls_batch *batch,
+                     const atg_bls_best *best);
+
+/* Device workspace atg_bls_run_best needs for a batch of this shape. */
+int atg_bls_workspace_size_best(const atg_ctx *ctx,
+                                const atg_bls_batch *batch, uint64_t *bytes);
+
 /* Top-k peaks of every curve's periodogram (asynchronous on the context
  * stream).  Replaces get_top_k_candidates (search/gpu_bls.py:146-174) as
  * BoxLeastSquaresGPU.power uses it (search/api.py:113-123): the local

@@ -40,6 +40,7 @@ def test_struct_layout_matches_header():
     from astro_transit_gpu_amd import _abi
     assert ctypes.sizeof(_abi.Batch) == 4 * 8 + 3 * 8 + 2 * 4 + 5 * 8 + 4 * 4
     assert ctypes.sizeof(_abi.Out) == 9 * 8
+    assert ctypes.sizeof(_abi.Best) == 10 * 8
     assert ctypes.sizeof(_abi.Tuning) == 16
 
 
@@ -61,6 +62,9 @@ def test_null_arguments_rejected(lib):
     lib.atg_bls_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
                                 ctypes.c_void_p]
     assert lib.atg_bls_run(None, None, None) == -1
+    lib.atg_bls_run_best.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p]
+    assert lib.atg_bls_run_best(None, None, None) == -1
     lib.atg_last_error.restype = ctypes.c_char_p
     lib.atg_last_error.argtypes = [ctypes.c_void_p]
     assert lib.atg_last_error(None) == b"null context"
