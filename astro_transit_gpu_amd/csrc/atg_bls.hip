@@ -290,9 +290,11 @@ __device__ int lower_key(const double *tn, int n, double P, double invP,
 // constant so that the C fold chains are independent straight-line code the
 // scheduler can interleave; the next point's loads are issued before the
 // current point is folded.  The near-integer test uses the curve's uniform
-// tolerances (tol_r, tol_x: bounds of the per-point ones of fold_bin), so a
-// superset of the uncertain bins is recomputed with true divisions before
-// the atomics and every other bin is known to lie in [1, nb - os].
+// tolerances (tol_r, tol_x: bounds of the per-point ones of fold_bin): an
+// unflagged point's reciprocal bin is the oracle's and lies in [1, nb - os];
+// a flagged point (a superset of the uncertain ones) is first added to its
+// reciprocal bin, within [1, nb], and then moved to the bin the oracle's
+// divisions give.
 template <int C, bool EXACT, bool PACKED>
 __device__ __forceinline__ void bin_points(
     const double *tp, const long long *wp, const long long *yp, int L,
@@ -302,6 +304,14 @@ __device__ __forceinline__ void bin_points(
     double tv_n = tp[0];
     long long wv_n = wp[0], yv_n = PACKED ? 0 : yp[0];
     const double hr = 1.0 - tol_r, hx = 1.0 - tol_x;
+    // per-period bin origins (bin 1 for the exact fold, bin 0 for the fast
+    // one), so that a bin's LDS address is one shift-and-add of its index
+    long long *ow[C], *oy[C];
+#pragma unroll
+    for (int s = 0; s < C; ++s) {
+        ow[s] = binW + base[s] + (EXACT ? 1 : 0);
+        oy[s] = binY + base[s] + (EXACT ? 1 : 0);
+    }
     for (int i = 0; i < L; ++i) {
         const double tv = tv_n;
         const long long wv = wv_n, yv = yv_n;
@@ -330,28 +340,61 @@ __device__ __forceinline__ void bin_points(
                 fl[s] = (fr <= tol_r) | (fr >= hr) | (fx <= tol_x) |
                         (fx >= hx);
                 any |= fl[s];
-                at[s] = base[s] + 1 + (int)x;
+                at[s] = (int)x;  // bin at[s] + 1
             } else {
                 // fast mode: FMA phase and rounded reciprocals only; a point
                 // within ~1e-15 of a bin edge may land in the neighbour bin
                 const double ph = fma(-per[s], floor(tv * inv[s]), tv);
-                at[s] = base[s] + min(max((int)(ph * inv_bd) + 1, 0), nb[s]);
+                at[s] = min(max((int)(ph * inv_bd) + 1, 0), nb[s]);
             }
-        }
-        if (EXACT && any) {
-#pragma unroll
-            for (int s = 0; s < C; ++s)
-                if (fl[s])
-                    at[s] = base[s] +
-                            min(max(fold_exact(tv, per[s], bd), 0), nb[s]);
         }
 #pragma unroll
         for (int s = 0; s < C; ++s) {
-            __hip_atomic_fetch_add(&binW[at[s]], wv, __ATOMIC_RELAXED,
+            __hip_atomic_fetch_add(ow[s] + at[s], wv, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
             if (!PACKED)
-                __hip_atomic_fetch_add(&binY[at[s]], yv, __ATOMIC_RELAXED,
+                __hip_atomic_fetch_add(oy[s] + at[s], yv, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        if (EXACT && any) {
+            // Every point went to its reciprocal bin above, so the atomics
+            // share the fold's basic block.  A flagged point whose oracle
+            // bin differs moves there: integer adds commute, so the
+            // histogram is the one the oracle's bins give.  The cycle
+            // division runs only where r itself is within tol_r of an
+            // integer (otherwise floor(r) is the oracle's floor(t/P)).  tx
+            // hides the folds above from common-subexpression elimination:
+            // keeping their values live into this rare path would cost
+            // registers in the main loop.
+            double tx = tv;
+            asm volatile("" : "+v"(tx));
+#pragma unroll
+            for (int s = 0; s < C; ++s)
+                if (fl[s]) {
+                    const double r = tx * inv[s];
+                    double q = floor(r);
+                    const double fr = r - q;
+                    const int a =
+                        base[s] + 1 + (int)((tx - per[s] * q) * inv_bd);
+                    if (fr <= tol_r || fr >= hr) q = floor(tx / per[s]);
+                    const double ph = tx - per[s] * q;
+                    const int e =
+                        base[s] + min(max((int)trunc(ph / bd) + 1, 0), nb[s]);
+                    if (e != a) {
+                        __hip_atomic_fetch_add(&binW[a], -wv, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __hip_atomic_fetch_add(&binW[e], wv, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (!PACKED) {
+                            __hip_atomic_fetch_add(
+                                &binY[a], -yv, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_WORKGROUP);
+                            __hip_atomic_fetch_add(
+                                &binY[e], yv, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                    }
+                }
         }
     }
 }
@@ -451,7 +494,7 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     P.n = n;
     P.n_dur = (int)nd;
     P.lc_self = lc;
-    P.ws_off = off + (long long)lc * NT;
+    P.ws_off = off + (long long)lc * (2 * NT);
     P.L = (n + NT - 1) / NT;
     int status = ATG_LC_OK;
     if (n <= 0 || np <= 0 || nd <= 0) status = ATG_LC_EMPTY;
@@ -592,20 +635,67 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     P.tol_r = ldexp((tmax - tmin) / pmin + 1.0, -49);
     P.tol_x = ldexp(ceil(pmax / P.bd) + b.oversample + 2.0, -49);
 
-    // pass C: lane-major workspace; lane j owns points [j*L, (j+1)*L)
+    // pass C: lane-major workspace (slot = step * NT + lane).  Points whose
+    // first-cycle bin quotient t'/bd lies at a bin edge (exactly gridded
+    // times put every few dozenth point there, and t' = 0 too) are flagged
+    // by the near-integer test of every period whose first cycle holds
+    // them, which sends their whole wave through the divisions.  They go to
+    // the last E_steps steps, lanes side by side, so that they share a few
+    // wave steps; every other point keeps the contiguous runs (lane j owns
+    // its slice of [j*L0, (j+1)*L0)).  The atomics are order-independent,
+    // so the placement changes nothing but the time.
+    const double inv_bd0 = 1.0 / P.bd, edge_tol = 4.0 * P.tol_x;
+    auto is_edge = [&](double tp) {
+        const double f = __builtin_amdgcn_fract(tp * inv_bd0);
+        return f <= edge_tol || f >= 1.0 - edge_tol;
+    };
+    const int Lr = P.L;  // read chunk: thread tid reads [tid*Lr, (tid+1)*Lr)
+    int my_edges = 0;
+    for (int i = 0; i < Lr; ++i) {
+        const int src = tid * Lr + i;
+        if (src < n && is_edge(b.t[off + src] - tmin)) ++my_edges;
+    }
+    int n_edge;
+    const int edge_before = block_excl_scan(my_edges, iscr, &n_edge);
+    const int n0 = n - n_edge;
+    const int L0 = (n0 + NT - 1) / NT, L1 = (n_edge + NT - 1) / NT;
+    P.L = L0 + L1;
     long long wq_sum = 0, yq_sum = 0;
-    const int L = P.L;
-    for (int i = 0; i < L; ++i) {
-        int src = tid * L + i;
-        int s = src < n ? src : n - 1;
-        double tp = b.t[off + s] - tmin;
-        long long wq = 0, yq = 0;
-        if (src < n) quantize(b, off + s, med, wscale, yscale, wq, yq);
-        long long slot = P.ws_off + (long long)i * NT + tid;
+    const double t_pad = tmax - tmin;
+    const long long yq_pad = 0;
+    // padding slots of this lane: zero weight, an in-range time
+    for (int i = 0; i < L0; ++i)
+        if ((long long)tid * L0 + i >= n0) {
+            const long long slot = P.ws_off + (long long)i * NT + tid;
+            w.ws_t[slot] = t_pad;
+            w.ws_w[slot] = 0;
+            w.ws_y[slot] = yq_pad;
+        }
+    for (int i = 0; i < L1; ++i)
+        if (i * NT + tid >= n_edge) {
+            const long long slot = P.ws_off + (long long)(L0 + i) * NT + tid;
+            w.ws_t[slot] = t_pad;
+            w.ws_w[slot] = 0;
+            w.ws_y[slot] = yq_pad;
+        }
+    int e_seen = 0;
+    for (int i = 0; i < Lr; ++i) {
+        const int src = tid * Lr + i;
+        if (src >= n) break;
+        const double tp = b.t[off + src] - tmin;
+        long long wq, yq;
+        quantize(b, off + src, med, wscale, yscale, wq, yq);
+        long long slot;
+        if (is_edge(tp)) {
+            const int e = edge_before + e_seen++;
+            slot = P.ws_off + (long long)(L0 + e / NT) * NT + e % NT;
+        } else {
+            const int d = src - edge_before - e_seen;  // rank among the rest
+            slot = P.ws_off + (long long)(d % L0) * NT + d / L0;
+        }
         w.ws_t[slot] = tp;
         // packed curves keep the packed bin increment in ws_w
-        w.ws_w[slot] = P.packed ? (src < n ? (1ll << P.ybits) : 0ll) + yq
-                                : wq;
+        w.ws_w[slot] = P.packed ? (1ll << P.ybits) + yq : wq;
         w.ws_y[slot] = yq;
         if (src == 0) P.wq_one = wq;
         wq_sum += wq;
@@ -697,10 +787,24 @@ struct Score {
             idx = i2;
         }
     }
-    // for a candidate whose index exceeds every one offered before
-    __device__ __forceinline__ void offer_later(double q2, double d2,
+    // offer() of a window that counts only if ok; returns whether it was
+    // taken
+    __device__ __forceinline__ bool offer_if(bool ok, double q2, double d2,
+                                             int i2) {
+        const double lhs = q2 * den, rhs = q * d2;
+        const bool take = ok & ((lhs > rhs) | ((lhs == rhs) & (i2 < idx)));
+        if (take) {
+            q = q2;
+            den = d2;
+            idx = i2;
+        }
+        return take;
+    }
+    // for a candidate whose index exceeds every one offered before; ok
+    // false leaves the score as it is
+    __device__ __forceinline__ void offer_later(bool ok, double q2, double d2,
                                                 int i2) {
-        if (q2 * den > q * d2) {
+        if (ok & (q2 * den > q * d2)) {
             q = q2;
             den = d2;
             idx = i2;
@@ -792,23 +896,23 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
     const bool like = b.objective == ATG_OBJ_LIKELIHOOD;
     Score best;
     // duration-major: a thread meets its windows in increasing index
-    // (k * nval + s), so a later equal score never replaces the held one,
-    // and the four prefix values of a window are independent loads
+    // (k * nval + s), so a later equal score never replaces the held one.
+    // The window tests are a predicate rather than branches, so that the
+    // unrolled iterations issue all their LDS loads before the first wait.
     for (int k = 0; k < nd; ++k) {
         const int m = s_m[k];
         const int send = min(nb - 1, nb - m);
-#pragma unroll 2
+#pragma unroll 4
         for (int s = tid; s <= send; s += NT) {
             const double w0 = Cw[base + s], y0 = Cy[base + s];
             const double we = Cw[base + s + m], ye = Cy[base + s + m];
             const double win = we - w0;
             const double yin = ye - y0;
             const double wout = 1.0 - win;
-            if (win < epsn || wout < epsn) continue;
             // depth * W_in * W_out / W_tot^2 = y_tot*w_in - y_in
             const double num = fma(ytn, win, -yin);
-            if (num < 0.0) continue;
-            best.offer_later(num * num,
+            const bool ok = !(win < epsn) & !(wout < epsn) & !(num < 0.0);
+            best.offer_later(ok, num * num,
                              like ? win * wout * wout : win * wout,
                              k * nval + s);
         }
@@ -1055,7 +1159,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
         for (int k = 0; k < nd; ++k) {
             const int m = s_m[k];
             const int last = min(nend - 1, nb - m);
-#pragma unroll 2
+#pragma unroll 4
             for (int n = s0 + tid; n <= last; n += NT) {
                 const int l = n - s0;
                 const double w0 = Cw[l], y0 = Cy[l];
@@ -1063,13 +1167,12 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
                 const double win = we - w0;
                 const double yin = ye - y0;
                 const double wout = 1.0 - win;
-                if (win < epsn || wout < epsn) continue;
                 const double num = fma(ytn, win, -yin);
-                if (num < 0.0) continue;
-                const int before = best.idx;
-                best.offer(num * num, like ? win * wout * wout : win * wout,
-                           k * nval + n);
-                if (best.idx != before) {
+                const bool ok =
+                    !(win < epsn) & !(wout < epsn) & !(num < 0.0);
+                if (best.offer_if(ok, num * num,
+                                  like ? win * wout * wout : win * wout,
+                                  k * nval + n)) {
                     bwin = win;
                     byin = yin;
                 }
@@ -2357,7 +2460,7 @@ struct Layout {
 Layout layout_of(const atg_bls_batch *b) {
     Layout L;
     const size_t nlc = (size_t)b->n_lc;
-    const size_t slots = (size_t)b->n_points + nlc * NT;
+    const size_t slots = (size_t)b->n_points + nlc * (2 * NT);
     size_t o = 0;
     L.prm = o;     o = align256(o + nlc * sizeof(LcParams));
     L.flags = o;   o = align256(o + nlc * sizeof(int));
