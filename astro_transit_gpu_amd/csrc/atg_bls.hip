@@ -706,7 +706,13 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     // pass D: point-order copies for the ranged phase tiles of periods
     // whose bins exceed the LDS histogram (time-sorted curves only)
     P.natural = 0;
-    if (P.sorted && (double)lds_bins < ceil(pmax / P.bd) + b.oversample + 1) {
+    // (k_main's test: packed bins take 12 bytes each in the scan, 16
+    // otherwise, out of lds_bins * 16)
+    const double nbmax = ceil(pmax / P.bd) + b.oversample;
+    const bool big = P.packed
+                         ? nbmax + 1 + floor((nbmax + 2) / 2) > 2.0 * lds_bins
+                         : nbmax + 1 > (double)lds_bins;
+    if (P.sorted && big) {
         P.natural = 1;
         for (int i = tid; i < n; i += NT) {
             long long wq, yq;
@@ -830,11 +836,43 @@ __device__ __forceinline__ void write_nan(const Out &o, long long p) {
     o.log_like[p] = nan;
 }
 
+// Sliding-box scan of one period's prefix arrays (Cw or the point counts
+// Cn, and Cy; index 0..nb).  Duration-major: a thread meets its windows in
+// increasing index (k * nval + s), so a later equal score never replaces
+// the held one.  The window tests are a predicate rather than branches, so
+// that the unrolled iterations issue all their LDS loads before the first
+// wait.
+template <bool PACKED>
+__device__ __forceinline__ void box_scan(Score &best, const double *Cw,
+                                         const int *Cn, const double *Cy,
+                                         double c1, const int *s_m, int nd,
+                                         int nb, double ytn, double epsn,
+                                         bool like) {
+    const int nval = nb + 1;
+    for (int k = 0; k < nd; ++k) {
+        const int m = s_m[k];
+        const int send = min(nb - 1, nb - m);
+#pragma unroll 4
+        for (int s = threadIdx.x; s <= send; s += NT) {
+            const double win =
+                PACKED ? (double)(Cn[s + m] - Cn[s]) * c1 : Cw[s + m] - Cw[s];
+            const double yin = Cy[s + m] - Cy[s];
+            const double wout = 1.0 - win;
+            // depth * W_in * W_out / W_tot^2 = y_tot*w_in - y_in
+            const double num = fma(ytn, win, -yin);
+            const bool ok = !(win < epsn) & !(wout < epsn) & !(num < 0.0);
+            best.offer_later(ok, num * num,
+                             like ? win * wout * wout : win * wout,
+                             k * nval + s);
+        }
+    }
+}
+
 // Padding, prefix scan and box scan of one period whose int64 histogram
 // occupies bins [base, base + nb] of binW / binY; writes period p.  A
-// packed histogram (P.packed) lives in binW alone; the caller then passes
-// binY = binW + nb + 1, the free space above it that receives the FP64
-// cumulative flux.
+// packed histogram (P.packed) lives in binW alone (binY is unused): its
+// scan writes the FP64 flux prefix in place and the int32 point counts into
+// the (nb + 2) / 2 slots above it.
 __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
                               long long p, double period, int nb, int base,
                               long long *binW, long long *binY,
@@ -869,23 +907,27 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
     long long totw, toty;
     long long ow = block_excl_scan(sw, lscr, &totw);
     long long oy = packed ? 0 : block_excl_scan(sy, lscr, &toty);
-    // cumulative sums in units of the total weight (FP64)
-    double *Cw = (double *)binW;
-    double *Cy = (double *)binY;
+    // cumulative sums in units of the total weight (FP64).  Packed bins:
+    // the flux prefix replaces the histogram in place and the weight prefix
+    // is kept as the int32 point count Cn just above it (12 bytes a bin);
+    // a window's weight is then its exact count times c1, the weight of one
+    // point.  Unpacked bins: weight and flux prefixes in place.
     const double fw = P.w_unit / P.W_tot, fy = P.y_unit / P.W_tot;
+    const double c1 = (double)P.wq_one * fw;
+    double *Cw = (double *)binW + base;
+    double *Cy = packed ? (double *)binW + base : (double *)binY + base;
+    int *Cn = (int *)(binW + base + nval);
     for (int i = b0; i < b1; ++i) {
         ow += binW[base + i];
         if (packed) {
-            // the packed prefix holds (points << ybits) + flux sum exactly;
-            // points * the common weight is the unpacked weight prefix
+            // the packed prefix holds (points << ybits) + flux sum exactly
             const long long ys = packed_y(ow, P.ybits);
-            const long long cnt = (ow - ys) >> P.ybits;
-            Cw[base + i] = (double)(cnt * P.wq_one) * fw;
-            Cy[base + i] = (double)ys * fy;
+            Cn[i] = (int)((ow - ys) >> P.ybits);
+            Cy[i] = (double)ys * fy;
         } else {
             oy += binY[base + i];
-            Cw[base + i] = (double)ow * fw;
-            Cy[base + i] = (double)oy * fy;
+            Cw[i] = (double)ow * fw;
+            Cy[i] = (double)oy * fy;
         }
     }
     __syncthreads();
@@ -895,28 +937,10 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
     const double ytn = Yt / Wt, epsn = DBL_EPSILON / Wt;
     const bool like = b.objective == ATG_OBJ_LIKELIHOOD;
     Score best;
-    // duration-major: a thread meets its windows in increasing index
-    // (k * nval + s), so a later equal score never replaces the held one.
-    // The window tests are a predicate rather than branches, so that the
-    // unrolled iterations issue all their LDS loads before the first wait.
-    for (int k = 0; k < nd; ++k) {
-        const int m = s_m[k];
-        const int send = min(nb - 1, nb - m);
-#pragma unroll 4
-        for (int s = tid; s <= send; s += NT) {
-            const double w0 = Cw[base + s], y0 = Cy[base + s];
-            const double we = Cw[base + s + m], ye = Cy[base + s + m];
-            const double win = we - w0;
-            const double yin = ye - y0;
-            const double wout = 1.0 - win;
-            // depth * W_in * W_out / W_tot^2 = y_tot*w_in - y_in
-            const double num = fma(ytn, win, -yin);
-            const bool ok = !(win < epsn) & !(wout < epsn) & !(num < 0.0);
-            best.offer_later(ok, num * num,
-                             like ? win * wout * wout : win * wout,
-                             k * nval + s);
-        }
-    }
+    if (packed)
+        box_scan<true>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn, epsn, like);
+    else
+        box_scan<false>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn, epsn, like);
     // workgroup argmax (highest score, then lowest (k, n))
     best.wave_reduce();
     const int wv = tid >> 6, l = tid & 63;
@@ -940,8 +964,10 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
             o.log_like[p] = 0.0;
         } else {
             const int k = bi / nval, s = bi - k * nval, m = s_m[k];
-            const double win = (Cw[base + s + m] - Cw[base + s]) * Wt;
-            const double yin_s = (Cy[base + s + m] - Cy[base + s]) * Wt;
+            const double win =
+                (packed ? (double)(Cn[s + m] - Cn[s]) * c1 : Cw[s + m] - Cw[s]) *
+                Wt;
+            const double yin_s = (Cy[s + m] - Cy[s]) * Wt;
             const double wout = Wt - win;
             const double y_in = yin_s / win;
             const double y_out = (Yt - yin_s) / wout;
@@ -1018,6 +1044,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
     const double Wt = P.W_tot, Yt = P.Y_tot;
     const double ytn = Yt / Wt, epsn = DBL_EPSILON / Wt;
     const double fw = P.w_unit / Wt, fy = P.y_unit / Wt;
+    const double c1 = (double)P.wq_one * (P.w_unit / P.W_tot);
     // ranged form: cycles 0..Q of the sorted points
     const double *tn = w.nat_t + b.lc_off[P.lc_self];
     int ncyc = 0;
@@ -1149,7 +1176,9 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
                 s_carry[0] = ow;
                 s_carry[1] = oy;
             }
-            Cw[i] = (double)ow * fw;
+            // packed curves: the point count, as in finish_period (every
+            // point adds the common weight wq_one)
+            Cw[i] = P.packed ? (double)(ow / P.wq_one) : (double)ow * fw;
             Cy[i] = (double)oy * fy;
         }
         __syncthreads();
@@ -1164,7 +1193,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
                 const int l = n - s0;
                 const double w0 = Cw[l], y0 = Cy[l];
                 const double we = Cw[l + m], ye = Cy[l + m];
-                const double win = we - w0;
+                const double win = P.packed ? (we - w0) * c1 : we - w0;
                 const double yin = ye - y0;
                 const double wout = 1.0 - win;
                 const double num = fma(ytn, win, -yin);
@@ -1325,8 +1354,8 @@ __global__ __launch_bounds__(NT) void k_main(Batch b, Out o, Work w,
         while (p < pend) {
             // greedy sub-batch of periods that fit in the LDS histogram.  A
             // packed histogram takes 8 B per bin, so both halves of the LDS
-            // hold bins; each packed period also needs nb + 1 free slots
-            // above it for its FP64 flux prefix (finished last to first).
+            // hold bins; each packed period also needs (nb + 2) / 2 free
+            // slots above it for its point counts (finished last to first).
             const bool packed = P.packed;
             const int cap = packed ? 2 * lds_bins : lds_bins;
             double per[SUBMAX], inv[SUBMAX];
@@ -1344,7 +1373,8 @@ __global__ __launch_bounds__(NT) void k_main(Batch b, Out o, Work w,
                 if (cnt == s && p + s < pend) {
                     const double ps = b.periods[p + s];
                     const int nbs = (int)ceil(ps / bd) + os;
-                    if (used + (packed ? 2 : 1) * (nbs + 1) <= cap) {
+                    if (used + (nbs + 1) + (packed ? (nbs + 2) / 2 : 0) <=
+                        cap) {
                         per[s] = ps;
                         inv[s] = 1.0 / ps;
                         nb[s] = nbs;
@@ -1411,9 +1441,8 @@ __global__ __launch_bounds__(NT) void k_main(Batch b, Out o, Work w,
                     const double ps = b.periods[p + s];
                     const int nbs = (int)ceil(ps / bd) + os;
                     const int bse = end - (nbs + 1);
-                    finish_period(b, o, P, p + s, ps, nbs, bse, binW,
-                                  binW + (nbs + 1), s_m, nd, lscr, dscr,
-                                  iscr);
+                    finish_period(b, o, P, p + s, ps, nbs, bse, binW, binW,
+                                  s_m, nd, lscr, dscr, iscr);
                     end = bse;
                 }
             } else {
