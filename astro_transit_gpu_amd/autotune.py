@@ -6,7 +6,7 @@ Candidates (each a legal launch on gfx950):
                       points and the threshold of the phase-tiled path;
   * periods_per_item  periods per work item of the persistent queue;
   * wg_per_cu         persistent workgroups per CU, limited by 160 KiB of LDS
-                      and by the kernel's register use (3 waves per SIMD).
+                      and by the kernel's register use (4 waves per SIMD).
 
 Each candidate runs the main kernel on a sample of the workload and is timed
 with HIP events on the context's stream; the fastest is applied and cached
@@ -25,11 +25,14 @@ from .engine import Context, DeviceBatch
 
 CACHE = os.path.join(LIB_DIR, "tuning.json")
 LDS_PER_CU = 160 * 1024
-STATIC_LDS = 1024          # upper bound of the kernel's static LDS
-MAX_WAVES_PER_SIMD = 3     # register-limited
+STATIC_LDS = 512           # upper bound of the kernel's static LDS
+MAX_WAVES_PER_SIMD = 4     # register-limited
+# bumped whenever the kernel's speed profile changes, so that cached
+# tilings of an older build are not reused
+KERNEL_GENERATION = 2
 
 
-def candidates(lds_options=(2048, 3072, 4096, 5120, 6144),
+def candidates(lds_options=(2048, 2432, 2528, 3072, 4096, 5120, 6144),
                ppi_options=(4, 8, 16)):
     out = []
     for lds in lds_options:
@@ -45,7 +48,8 @@ def candidates(lds_options=(2048, 3072, 4096, 5120, 6144),
 def signature(n_lc, n_points_mean, bins_mean, n_dur, mode):
     """Workload key: buckets wide enough that nearby shapes share a
     tiling."""
-    return "lc%d_n%d_b%d_d%d_%s" % (
+    return "g%d_lc%d_n%d_b%d_d%d_%s" % (
+        KERNEL_GENERATION,
         int(np.log2(max(n_lc, 1))), int(np.log2(max(n_points_mean, 1))),
         int(np.log2(max(bins_mean, 1))), n_dur, mode)
 

@@ -1284,7 +1284,11 @@ __device__ __forceinline__ void flush_run(long long *binW, long long *binY,
     }
 }
 
-__global__ __launch_bounds__(NT) void k_main(Batch b, Out o, Work w,
+// Four waves per SIMD (at most 128 VGPRs): the fold, the LDS atomics and
+// the box scan's loads are latency-bound at three; four workgroups of
+// lds_bins <= 2528 share a CU's 160 KiB of LDS.
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void k_main(Batch b, Out o, Work w,
                                              int per_item, int lds_bins) {
     extern __shared__ __align__(16) unsigned char smem[];
     long long *binW = (long long *)smem;
@@ -2456,7 +2460,7 @@ struct atg_ctx {
     std::string err;
     void *ws = nullptr;
     size_t ws_bytes = 0;
-    atg_tuning tune{8, 3072, 3, 0};
+    atg_tuning tune{8, 2432, 4, 0};
     int num_cu = 0;
     size_t lds_max = 65536;
     size_t lds_attr_set = 65536;
