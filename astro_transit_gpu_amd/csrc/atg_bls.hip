@@ -842,12 +842,11 @@ __device__ __forceinline__ void write_nan(const Out &o, long long p) {
 // the held one.  The window tests are a predicate rather than branches, so
 // that the unrolled iterations issue all their LDS loads before the first
 // wait.
-template <bool PACKED>
+template <bool PACKED, bool LIKE>
 __device__ __forceinline__ void box_scan(Score &best, const double *Cw,
                                          const int *Cn, const double *Cy,
                                          double c1, const int *s_m, int nd,
-                                         int nb, double ytn, double epsn,
-                                         bool like) {
+                                         int nb, double ytn, double epsn) {
     const int nval = nb + 1;
     for (int k = 0; k < nd; ++k) {
         const int m = s_m[k];
@@ -862,7 +861,7 @@ __device__ __forceinline__ void box_scan(Score &best, const double *Cw,
             const double num = fma(ytn, win, -yin);
             const bool ok = !(win < epsn) & !(wout < epsn) & !(num < 0.0);
             best.offer_later(ok, num * num,
-                             like ? win * wout * wout : win * wout,
+                             LIKE ? win * wout * wout : win * wout,
                              k * nval + s);
         }
     }
@@ -937,10 +936,18 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
     const double ytn = Yt / Wt, epsn = DBL_EPSILON / Wt;
     const bool like = b.objective == ATG_OBJ_LIKELIHOOD;
     Score best;
-    if (packed)
-        box_scan<true>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn, epsn, like);
-    else
-        box_scan<false>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn, epsn, like);
+    if (packed) {
+        if (like)
+            box_scan<true, true>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn, epsn);
+        else
+            box_scan<true, false>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn, epsn);
+    } else {
+        if (like)
+            box_scan<false, true>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn, epsn);
+        else
+            box_scan<false, false>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn,
+                                   epsn);
+    }
     // workgroup argmax (highest score, then lowest (k, n))
     best.wave_reduce();
     const int wv = tid >> 6, l = tid & 63;
