@@ -125,6 +125,10 @@ struct Work {
     long long *nat_y;     // bins exceed the LDS histogram
     unsigned int *cand_count;  // periods queued for refinement
     unsigned int *cand_next;   // refinement queue head
+    unsigned int *big_count;   // periods left to k_big (bins exceed LDS)
+    unsigned int *big_next;    // k_big queue head
+    long long *big_p;          // [n_periods] those periods ...
+    int *big_lc;               // ... and their curves
     int *cand_lc;         // [n_lc * CAP] curve of each queued period
     long long *cand_p;    // [n_lc * CAP] its global period index
 };
@@ -301,8 +305,12 @@ __device__ __forceinline__ void bin_points(
     const double *per, const double *inv, const int *nb, const int *base,
     double bd, double inv_bd, double tol_r, double tol_x, long long *binW,
     long long *binY) {
-    double tv_n = tp[0];
-    long long wv_n = wp[0], yv_n = PACKED ? 0 : yp[0];
+    // tp, wp, yp: the curve's (uniform) slot bases; lane tid reads slots
+    // tid + i * NT through a 32-bit offset, so that the loads address
+    // base + offset and no 64-bit per-lane pointer stays live
+    unsigned int sl = threadIdx.x;
+    double tv_n = tp[sl];
+    long long wv_n = wp[sl], yv_n = PACKED ? 0 : yp[sl];
     const double hr = 1.0 - tol_r, hx = 1.0 - tol_x;
     // per-period bin origins (bin 1 for the exact fold, bin 0 for the fast
     // one), so that a bin's LDS address is one shift-and-add of its index
@@ -316,9 +324,10 @@ __device__ __forceinline__ void bin_points(
         const double tv = tv_n;
         const long long wv = wv_n, yv = yv_n;
         if (i + 1 < L) {
-            tv_n = tp[(long long)(i + 1) * NT];
-            wv_n = wp[(long long)(i + 1) * NT];
-            if (!PACKED) yv_n = yp[(long long)(i + 1) * NT];
+            sl += NT;
+            tv_n = tp[sl];
+            wv_n = wp[sl];
+            if (!PACKED) yv_n = yp[sl];
         }
         int at[C];
         bool fl[C];
@@ -762,6 +771,8 @@ __global__ __launch_bounds__(NT) void k_items(long long n_lc,
         *w.counter = 0u;
         *w.cand_count = 0u;
         *w.cand_next = 0u;
+        *w.big_count = 0u;
+        *w.big_next = 0u;
     }
 }
 
@@ -1304,9 +1315,8 @@ void k_main(Batch b, Out o, Work w,
     __shared__ unsigned int s_item;
     __shared__ int s_queue;
     __shared__ long long lscr[NW];
-    __shared__ double dscr[2 * NW], dscr2[NW], dscr3[NW];
+    __shared__ double dscr[2 * NW];
     __shared__ int iscr[NW];
-    __shared__ long long s_carry[2];
 
     const int tid = threadIdx.x;
     const long long n_lc = b.n_lc;
@@ -1355,8 +1365,6 @@ void k_main(Batch b, Out o, Work w,
         const long long dof = b.dur_off[lc];
         if (tid < nd) s_m[tid] = (int)round(b.durs[dof + tid] / P.bd);
         __syncthreads();
-        int mmax = 0;
-        for (int k = 0; k < nd; ++k) mmax = max(mmax, s_m[k]);
         const double bd = P.bd;
         const double inv_bd = 1.0 / bd;
         const int os = b.oversample;
@@ -1395,15 +1403,14 @@ void k_main(Batch b, Out o, Work w,
                     }
                 }
             }
-            if (cnt == 0) {  // this period alone exceeds the LDS histogram
-                const int nbp = (int)ceil(b.periods[p] / bd) + os;
-                if (nbp - os > os && lds_bins - mmax - 1 >= 1) {
-                    big_period(b, o, w, P, p, nbp, lds_bins, binW, binY, s_m,
-                               nd, mmax, lscr, dscr, iscr, dscr2, dscr3,
-                               s_carry);
-                } else if (tid == 0) {
-                    write_nan(o, p);
-                    atomicOr(&w.lc_flags[lc], 1);
+            if (cnt == 0) {  // this period alone exceeds the LDS histogram:
+                             // k_big takes it (its phase tiles need more
+                             // registers than this kernel's four waves
+                             // per SIMD leave)
+                if (tid == 0) {
+                    const unsigned int j = atomicAdd(w.big_count, 1u);
+                    w.big_p[j] = p;
+                    w.big_lc[j] = lc;
                 }
                 ++p;
                 continue;
@@ -1419,9 +1426,10 @@ void k_main(Batch b, Out o, Work w,
             // their bins rarely collide and every point goes straight to
             // the LDS histogram (padding slots carry zero weights)
             {
-                const double *tp = w.ws_t + P.ws_off + tid;
-                const long long *wp = w.ws_w + P.ws_off + tid;
-                const long long *yp = w.ws_y + P.ws_off + tid;
+                // uniform bases; each lane adds its 32-bit slot offset
+                const double *tp = w.ws_t + P.ws_off;
+                const long long *wp = w.ws_w + P.ws_off;
+                const long long *yp = w.ws_y + P.ws_off;
                 const int L = P.L;
 #define ATG_BIN(C, E, K)                                                    \
     case C:                                                                 \
@@ -1472,6 +1480,51 @@ void k_main(Batch b, Out o, Work w,
 }
 
 }  // namespace
+
+// ------------------------------------------------------------------- k_big
+
+// Periods whose bins exceed the LDS histogram, queued by k_main, phase tile
+// by phase tile (big_period); persistent workgroups pull them from the
+// list, so that a workload of many long periods stays balanced.
+__global__ __launch_bounds__(NT) void k_big(Batch b, Out o, Work w,
+                                            int lds_bins) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    long long *binW = (long long *)smem;
+    long long *binY = binW + lds_bins;
+    __shared__ int s_m[DMAX];
+    __shared__ unsigned int s_j;
+    __shared__ long long lscr[NW];
+    __shared__ double dscr[2 * NW], dscr2[NW], dscr3[NW];
+    __shared__ int iscr[NW];
+    __shared__ long long s_carry[2];
+    const int tid = threadIdx.x;
+    const unsigned int total = *w.big_count;
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) s_j = atomicAdd(w.big_next, 1u);
+        __syncthreads();
+        const unsigned int j = s_j;
+        if (j >= total) break;
+        const long long p = w.big_p[j];
+        const int lc = w.big_lc[j];
+        const LcParams P = w.prm[lc];
+        const int nd = P.n_dur;
+        const long long dof = b.dur_off[lc];
+        if (tid < nd) s_m[tid] = (int)round(b.durs[dof + tid] / P.bd);
+        __syncthreads();
+        int mmax = 0;
+        for (int k = 0; k < nd; ++k) mmax = max(mmax, s_m[k]);
+        const int os = b.oversample;
+        const int nbp = (int)ceil(b.periods[p] / P.bd) + os;
+        if (nbp - os > os && lds_bins - mmax - 1 >= 1) {
+            big_period(b, o, w, P, p, nbp, lds_bins, binW, binY, s_m, nd,
+                       mmax, lscr, dscr, iscr, dscr2, dscr3, s_carry);
+        } else if (tid == 0) {
+            write_nan(o, p);
+            atomicOr(&w.lc_flags[lc], 1);
+        }
+    }
+}
 
 // ------------------------------------------------------- k_cands / k_exact
 //
@@ -2467,7 +2520,7 @@ struct atg_ctx {
     std::string err;
     void *ws = nullptr;
     size_t ws_bytes = 0;
-    atg_tuning tune{8, 2432, 4, 0};
+    atg_tuning tune{16, 2528, 4, 0};
     int num_cu = 0;
     size_t lds_max = 65536;
     size_t lds_attr_set = 65536;
@@ -2494,7 +2547,7 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct Layout {
     size_t prm, flags, items, queues, counter, cand_lc, cand_p, t, w, y,
-        nt, nw, ny, tk_pos, tk_st, total;
+        nt, nw, ny, tk_pos, tk_st, big_p, big_lc, total;
 };
 
 Layout layout_of(const atg_bls_batch *b) {
@@ -2506,7 +2559,7 @@ Layout layout_of(const atg_bls_batch *b) {
     L.flags = o;   o = align256(o + nlc * sizeof(int));
     L.items = o;   o = align256(o + (nlc + 1) * sizeof(long long));
     L.queues = o;  o = align256(o + NQ * (sizeof(long long) + sizeof(int)));
-    L.counter = o; o = align256(o + 3 * sizeof(unsigned int));
+    L.counter = o; o = align256(o + 5 * sizeof(unsigned int));
     L.cand_lc = o; o = align256(o + nlc * CAP * sizeof(int));
     L.cand_p = o;  o = align256(o + nlc * CAP * sizeof(long long));
     L.t = o;       o = align256(o + slots * sizeof(double));
@@ -2519,6 +2572,8 @@ Layout layout_of(const atg_bls_batch *b) {
     const size_t nper = (size_t)b->n_periods;
     L.tk_pos = o;  o = align256(o + nper * sizeof(int));
     L.tk_st = o;   o = align256(o + nper);
+    L.big_p = o;   o = align256(o + nper * sizeof(long long));
+    L.big_lc = o;  o = align256(o + nper * sizeof(int));
     L.total = o;
     return L;
 }
@@ -2626,6 +2681,10 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
     w.nat_y = (long long *)(base + L.ny);
     w.cand_count = w.counter + 1;
     w.cand_next = w.counter + 2;
+    w.big_count = w.counter + 3;
+    w.big_next = w.counter + 4;
+    w.big_p = (long long *)(base + L.big_p);
+    w.big_lc = (int *)(base + L.big_lc);
     w.cand_lc = (int *)(base + L.cand_lc);
     w.cand_p = (long long *)(base + L.cand_p);
     Batch kb = to_batch(b);
@@ -2668,10 +2727,11 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
     const int nlc = (int)b->n_lc;
     const size_t lds = (size_t)c->tune.lds_bins * 16;
     if (lds > c->lds_attr_set) {
-        e = hipFuncSetAttribute((const void *)k_main,
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
-        if (e != hipSuccess) return hip_fail(c, e, "hipFuncSetAttribute");
+        for (const void *f : {(const void *)k_main, (const void *)k_big}) {
+            e = hipFuncSetAttribute(
+                f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return hip_fail(c, e, "hipFuncSetAttribute");
+        }
         c->lds_attr_set = lds;
     }
     hipLaunchKernelGGL(k_prep, dim3(nlc), dim3(NT), 0, c->stream, kb, w,
@@ -2700,6 +2760,8 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
         hipEventRecord(ev1, c->stream);
         c->pending.emplace_back(ev0, ev1);
     }
+    hipLaunchKernelGGL(k_big, dim3(grid), dim3(NT), lds, c->stream, kb, ko, w,
+                       c->tune.lds_bins);
     hipLaunchKernelGGL(k_cands, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w);
     const int ex_bins = std::min(c->tune.lds_bins, EXACT_BINS);
     hipLaunchKernelGGL(k_exact, dim3(c->num_cu * EXACT_WG_PER_CU), dim3(64),
