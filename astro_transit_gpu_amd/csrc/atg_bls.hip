@@ -125,8 +125,12 @@ struct Work {
     long long *nat_y;     // bins exceed the LDS histogram
     unsigned int *cand_count;  // periods queued for refinement
     unsigned int *cand_next;   // refinement queue head
-    unsigned int *big_count;   // periods left to k_big (bins exceed LDS)
-    unsigned int *big_next;    // k_big queue head
+    // periods left to k_big (bins exceed LDS), in NQ lists, one per
+    // queue: list q holds at most the periods of queue q's curves, from
+    // big_base[q] on
+    long long *big_base;       // [NQ]
+    unsigned int *big_count;   // [NQ] entries of each list
+    unsigned int *big_next;    // [NQ] k_big's heads
     long long *big_p;          // [n_periods] those periods ...
     int *big_lc;               // ... and their curves
     int *cand_lc;         // [n_lc * CAP] curve of each queued period
@@ -748,31 +752,37 @@ __global__ __launch_bounds__(NT) void k_items(long long n_lc,
                                               const long long *per_off,
                                               int per_item, Work w) {
     __shared__ long long lscr[NW];
+    long long pbase = 0;  // periods of the queues before q
     for (int q = 0; q < NQ; ++q) {
         const long long nq = (n_lc - q + NQ - 1) / NQ;
-        long long carry = 0;
+        long long carry = 0, pq = 0;
         for (long long base = 0; base < nq; base += NT) {
             const long long j = base + threadIdx.x;
             const long long lc = q + NQ * j;
-            long long c = 0;
-            if (j < nq)
-                c = (per_off[lc + 1] - per_off[lc] + per_item - 1) / per_item;
+            long long c = 0, np = 0;
+            if (j < nq) {
+                np = per_off[lc + 1] - per_off[lc];
+                c = (np + per_item - 1) / per_item;
+            }
             long long tot;
             const long long e = block_excl_scan(c, lscr, &tot);
             if (j < nq) w.item_off[lc] = carry + e;
             carry += tot;
+            pq += block_reduce(np, AddOp<long long>(), lscr);
         }
         if (threadIdx.x == 0) {
             w.qtot[q] = carry;
             w.qnext[q] = 0u;
+            w.big_base[q] = pbase;
+            w.big_count[q] = 0u;
+            w.big_next[q] = 0u;
         }
+        pbase += pq;
     }
     if (threadIdx.x == 0) {
         *w.counter = 0u;
         *w.cand_count = 0u;
         *w.cand_next = 0u;
-        *w.big_count = 0u;
-        *w.big_next = 0u;
     }
 }
 
@@ -1408,7 +1418,9 @@ void k_main(Batch b, Out o, Work w,
                              // registers than this kernel's four waves
                              // per SIMD leave)
                 if (tid == 0) {
-                    const unsigned int j = atomicAdd(w.big_count, 1u);
+                    const int q = lc % NQ;
+                    const long long j =
+                        w.big_base[q] + atomicAdd(&w.big_count[q], 1u);
                     w.big_p[j] = p;
                     w.big_lc[j] = lc;
                 }
@@ -1484,10 +1496,11 @@ void k_main(Batch b, Out o, Work w,
 // ------------------------------------------------------------------- k_big
 
 // Periods whose bins exceed the LDS histogram, queued by k_main, phase tile
-// by phase tile (big_period); persistent workgroups pull them from the
-// list, so that a workload of many long periods stays balanced.
-__global__ __launch_bounds__(NT) void k_big(Batch b, Out o, Work w,
-                                            int lds_bins) {
+// by phase tile (big_period).  Persistent workgroups drain the list of
+// their own XCD's queue first (the curve's points are in that XCD's L2),
+// then the others, so that a workload of many long periods stays balanced.
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void k_big(Batch b, Out o, Work w, int lds_bins) {
     extern __shared__ __align__(16) unsigned char smem[];
     long long *binW = (long long *)smem;
     long long *binY = binW + lds_bins;
@@ -1498,15 +1511,21 @@ __global__ __launch_bounds__(NT) void k_big(Batch b, Out o, Work w,
     __shared__ int iscr[NW];
     __shared__ long long s_carry[2];
     const int tid = threadIdx.x;
-    const unsigned int total = *w.big_count;
-    for (;;) {
+    int q = blockIdx.x % NQ, empty = 0;
+    while (empty < NQ) {
         __syncthreads();
-        if (tid == 0) s_j = atomicAdd(w.big_next, 1u);
+        if (tid == 0) s_j = atomicAdd(&w.big_next[q], 1u);
         __syncthreads();
         const unsigned int j = s_j;
-        if (j >= total) break;
-        const long long p = w.big_p[j];
-        const int lc = w.big_lc[j];
+        if (j >= w.big_count[q]) {  // list q is done (k_main has ended)
+            q = (q + 1) % NQ;
+            ++empty;
+            continue;
+        }
+        empty = 0;
+        const long long e = w.big_base[q] + j;
+        const long long p = w.big_p[e];
+        const int lc = w.big_lc[e];
         const LcParams P = w.prm[lc];
         const int nd = P.n_dur;
         const long long dof = b.dur_off[lc];
@@ -2547,7 +2566,7 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct Layout {
     size_t prm, flags, items, queues, counter, cand_lc, cand_p, t, w, y,
-        nt, nw, ny, tk_pos, tk_st, big_p, big_lc, total;
+        nt, nw, ny, tk_pos, tk_st, big_q, big_p, big_lc, total;
 };
 
 Layout layout_of(const atg_bls_batch *b) {
@@ -2559,7 +2578,8 @@ Layout layout_of(const atg_bls_batch *b) {
     L.flags = o;   o = align256(o + nlc * sizeof(int));
     L.items = o;   o = align256(o + (nlc + 1) * sizeof(long long));
     L.queues = o;  o = align256(o + NQ * (sizeof(long long) + sizeof(int)));
-    L.counter = o; o = align256(o + 5 * sizeof(unsigned int));
+    L.counter = o; o = align256(o + 3 * sizeof(unsigned int));
+    L.big_q = o;   o = align256(o + NQ * (sizeof(long long) + 2 * sizeof(int)));
     L.cand_lc = o; o = align256(o + nlc * CAP * sizeof(int));
     L.cand_p = o;  o = align256(o + nlc * CAP * sizeof(long long));
     L.t = o;       o = align256(o + slots * sizeof(double));
@@ -2681,8 +2701,9 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
     w.nat_y = (long long *)(base + L.ny);
     w.cand_count = w.counter + 1;
     w.cand_next = w.counter + 2;
-    w.big_count = w.counter + 3;
-    w.big_next = w.counter + 4;
+    w.big_base = (long long *)(base + L.big_q);
+    w.big_count = (unsigned int *)(base + L.big_q + NQ * sizeof(long long));
+    w.big_next = w.big_count + NQ;
     w.big_p = (long long *)(base + L.big_p);
     w.big_lc = (int *)(base + L.big_lc);
     w.cand_lc = (int *)(base + L.cand_lc);
@@ -2756,12 +2777,12 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
     const int grid = c->num_cu * c->tune.wg_per_cu;
     hipLaunchKernelGGL(k_main, dim3(grid), dim3(NT), lds, c->stream, kb, ko,
                        w, c->tune.periods_per_item, c->tune.lds_bins);
-    if (c->timing) {
+    hipLaunchKernelGGL(k_big, dim3(grid), dim3(NT), lds, c->stream, kb, ko, w,
+                       c->tune.lds_bins);
+    if (c->timing) {  // the periodogram stage: k_main and k_big
         hipEventRecord(ev1, c->stream);
         c->pending.emplace_back(ev0, ev1);
     }
-    hipLaunchKernelGGL(k_big, dim3(grid), dim3(NT), lds, c->stream, kb, ko, w,
-                       c->tune.lds_bins);
     hipLaunchKernelGGL(k_cands, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w);
     const int ex_bins = std::min(c->tune.lds_bins, EXACT_BINS);
     hipLaunchKernelGGL(k_exact, dim3(c->num_cu * EXACT_WG_PER_CU), dim3(64),
