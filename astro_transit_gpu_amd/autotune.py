@@ -25,14 +25,14 @@ from .engine import Context, DeviceBatch
 
 CACHE = os.path.join(LIB_DIR, "tuning.json")
 LDS_PER_CU = 160 * 1024
-STATIC_LDS = 512           # upper bound of the kernel's static LDS
+STATIC_LDS = 256           # the main kernel's static LDS
 MAX_WAVES_PER_SIMD = 4     # register-limited
 # bumped whenever the kernel's speed profile changes, so that cached
 # tilings of an older build are not reused
-KERNEL_GENERATION = 2
+KERNEL_GENERATION = 3
 
 
-def candidates(lds_options=(2048, 2432, 2528, 3072, 4096, 5120, 6144),
+def candidates(lds_options=(2048, 2432, 2540, 3072, 4096, 5120, 6144),
                ppi_options=(4, 8, 16)):
     out = []
     for lds in lds_options:

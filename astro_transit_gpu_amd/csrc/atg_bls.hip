@@ -59,6 +59,7 @@ constexpr int NT = 256;      // threads per workgroup = 4 wave64
 constexpr int NW = NT / 64;  // waves per workgroup
 constexpr int SUBMAX = 8;    // periods folded together in one pass
 constexpr int DMAX = 64;     // durations per light curve
+constexpr int EXACT_BINS = 1024;
 constexpr int NQ = 8;        // work queues = XCDs of an MI355X
 
 struct LcParams {
@@ -716,16 +717,19 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     }
     wq_sum = block_reduce(wq_sum, AddOp<long long>(), lscr);
     yq_sum = block_reduce(yq_sum, AddOp<long long>(), lscr);
-    // pass D: point-order copies for the ranged phase tiles of periods
-    // whose bins exceed the LDS histogram (time-sorted curves only)
+    // pass D: point-order copies for the ranged phase tiles (time-sorted
+    // curves only): of k_big, for periods whose bins exceed k_main's
+    // histogram (packed bins take 12 bytes each in its scan, 16 otherwise,
+    // out of lds_bins * 16), and of k_exact in parity mode, whose histogram
+    // holds at most EXACT_BINS bins
     P.natural = 0;
-    // (k_main's test: packed bins take 12 bytes each in the scan, 16
-    // otherwise, out of lds_bins * 16)
     const double nbmax = ceil(pmax / P.bd) + b.oversample;
     const bool big = P.packed
                          ? nbmax + 1 + floor((nbmax + 2) / 2) > 2.0 * lds_bins
                          : nbmax + 1 > (double)lds_bins;
-    if (P.sorted && big) {
+    const bool big_exact = b.mode == ATG_MODE_PARITY &&
+                           nbmax + 1 > (double)min(lds_bins, EXACT_BINS);
+    if (P.sorted && (big || big_exact)) {
         P.natural = 1;
         for (int i = tid; i < n; i += NT) {
             long long wq, yq;
@@ -866,7 +870,7 @@ __device__ __forceinline__ void write_nan(const Out &o, long long p) {
 template <bool PACKED, bool LIKE>
 __device__ __forceinline__ void box_scan(Score &best, const double *Cw,
                                          const int *Cn, const double *Cy,
-                                         double c1, const int *s_m, int nd,
+                                         double c1, const short *s_m, int nd,
                                          int nb, double ytn, double epsn) {
     const int nval = nb + 1;
     for (int k = 0; k < nd; ++k) {
@@ -896,7 +900,7 @@ __device__ __forceinline__ void box_scan(Score &best, const double *Cw,
 __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
                               long long p, double period, int nb, int base,
                               long long *binW, long long *binY,
-                              const int *s_m, int nd, long long *lscr,
+                              const short *s_m, int nd, long long *lscr,
                               double *dscr, int *iscr) {
     const int tid = threadIdx.x;
     const int os = b.oversample;
@@ -1321,7 +1325,10 @@ void k_main(Batch b, Out o, Work w,
     extern __shared__ __align__(16) unsigned char smem[];
     long long *binW = (long long *)smem;
     long long *binY = binW + lds_bins;
-    __shared__ int s_m[DMAX];
+    // duration bins; int16 is exact here: this kernel takes only periods of
+    // at most 2 * lds_bins < 32767 bins, and a window of m >= nb bins has
+    // no start, so larger m are clamped without effect
+    __shared__ short s_m[DMAX];
     __shared__ unsigned int s_item;
     __shared__ int s_queue;
     __shared__ long long lscr[NW];
@@ -1373,7 +1380,8 @@ void k_main(Batch b, Out o, Work w,
         }
         const int nd = P.n_dur;
         const long long dof = b.dur_off[lc];
-        if (tid < nd) s_m[tid] = (int)round(b.durs[dof + tid] / P.bd);
+        if (tid < nd)
+            s_m[tid] = (short)fmin(round(b.durs[dof + tid] / P.bd), 32767.0);
         __syncthreads();
         const double bd = P.bd;
         const double inv_bd = 1.0 / bd;
@@ -1568,7 +1576,6 @@ constexpr double TIE_REL = 1e-9;   // >> relative error of the int64 path
 // k_exact's FP64 histogram: one wave per queued period walks its points
 // sequentially, so the kernel is latency-bound; a small LDS histogram (16 KB)
 // lets 8 such waves share a CU (longer periods take the ranged segments).
-constexpr int EXACT_BINS = 1024;
 constexpr int EXACT_WG_PER_CU = 8;
 
 __global__ __launch_bounds__(NT) void k_cands(Batch b, Out o, Work w) {
@@ -2539,7 +2546,7 @@ struct atg_ctx {
     std::string err;
     void *ws = nullptr;
     size_t ws_bytes = 0;
-    atg_tuning tune{16, 2528, 4, 0};
+    atg_tuning tune{16, 2540, 4, 0};
     int num_cu = 0;
     size_t lds_max = 65536;
     size_t lds_attr_set = 65536;
@@ -2777,8 +2784,16 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
     const int grid = c->num_cu * c->tune.wg_per_cu;
     hipLaunchKernelGGL(k_main, dim3(grid), dim3(NT), lds, c->stream, kb, ko,
                        w, c->tune.periods_per_item, c->tune.lds_bins);
-    hipLaunchKernelGGL(k_big, dim3(grid), dim3(NT), lds, c->stream, kb, ko, w,
-                       c->tune.lds_bins);
+    // k_big's histogram is 32 bins (512 B) smaller where that is what its
+    // static LDS needs for as many workgroups per CU as k_main (its results
+    // do not depend on the tile size)
+    const int big_bins =
+        ((size_t)c->tune.lds_bins * 16 + 512) * c->tune.wg_per_cu >
+                160 * 1024
+            ? c->tune.lds_bins - 32
+            : c->tune.lds_bins;
+    hipLaunchKernelGGL(k_big, dim3(grid), dim3(NT), (size_t)big_bins * 16,
+                       c->stream, kb, ko, w, big_bins);
     if (c->timing) {  // the periodogram stage: k_main and k_big
         hipEventRecord(ev1, c->stream);
         c->pending.emplace_back(ev0, ev1);

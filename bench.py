@@ -363,8 +363,13 @@ def main():
                     help="curves of the ragged-batch leg")
     ap.add_argument("--serve5-seconds", type=float, default=3.0)
     ap.add_argument("--skip-extras", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tile the headline leg with the autotuner (default: "
+                    "the library's default tiling, the one the committed "
+                    "rocprofv3 summaries profile; the ragged leg always "
+                    "autotunes)")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the library's default tiling")
+                    help="accepted for older command lines (the default)")
     ap.add_argument("--precision", action="store_true",
                     help="run only the long-baseline precision study")
     ap.add_argument("--parity-sweep", type=int, default=0, metavar="N",
@@ -407,7 +412,7 @@ def main():
         del src, dst
     from astro_transit_gpu_amd.autotune import autotune
     t0 = time.perf_counter()
-    if args.no_autotune:
+    if not args.autotune:
         tuning = ctx.get_tuning()
     else:
         tuning, _ = autotune(ctx, d["t"], d["y"], d["dy"], d["periods"],
