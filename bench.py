@@ -59,7 +59,7 @@ def measured_traffic(n_lc):
     the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel
     calibrated), scaled to n_lc curves."""
     path = os.path.join(ROOT, "profiles",
-                        "k_main_hbm_traffic_1000x15k_5000p_5d.md")
+                        "k_main_k_big_hbm_traffic_1000x15k_5000p_5d.md")
     try:
         kv = dict(line.strip().split("=", 1) for line in open(path)
                   if "=" in line and line.strip().split("=", 1)[0] in
@@ -473,11 +473,11 @@ def main():
                     "frac": round(ach_lds / PEAK_LDS, 4),
                     "bytes_per_launch": per_launch_lds},
             # the survey's minimum (inputs 16 B/point, outputs) and what
-            # this design moves once: packed lane-major points 16 B/point,
-            # point-order copies for the phase tiles 24 B/point, outputs
+            # the stage reads once by design: the packed lane-major points
+            # (16 B/point; every period of this workload fits the LDS
+            # histogram, so no point-order reads), outputs
             "hbm_algorithmic_bytes_per_launch": hbm_in + hbm_out,
-            "hbm_design_bytes_per_launch": args.n_lc * 15000 * (16 + 24) +
-            hbm_out,
+            "hbm_design_bytes_per_launch": args.n_lc * 15000 * 16 + hbm_out,
         },
     }
     if not args.skip_extras:
