@@ -309,7 +309,7 @@ __device__ __forceinline__ void bin_points(
     const double *tp, const long long *wp, const long long *yp, int L,
     const double *per, const double *inv, const int *nb, const int *base,
     double bd, double inv_bd, double tol_r, double tol_x, long long *binW,
-    long long *binY) {
+    long long *binY, const double *pg) {
     // tp, wp, yp: the curve's (uniform) slot bases; lane tid reads slots
     // tid + i * NT through a 32-bit offset, so that the loads address
     // base + offset and no 64-bit per-lane pointer stays live
@@ -337,17 +337,22 @@ __device__ __forceinline__ void bin_points(
         int at[C];
         bool fl[C];
         bool any = false;
+        // t'/bd, shared by the C periods of the pass
+        const double tb = EXACT ? tv * inv_bd : 0.0;
 #pragma unroll
         for (int s = 0; s < C; ++s) {
             if (EXACT) {
                 // bin from the rounded reciprocals; a quotient within the
                 // uniform tolerance of an integer is flagged for the
                 // oracle's divisions.  Unflagged bins lie in [1, nb - os].
+                // per[s] holds P/bd here: x = t'/bd - q*P/bd is within
+                // u*(5*t'/bd + 6*P/bd) of the oracle's (t' - P*q)/bd (u =
+                // 2^-53: three roundings of the products, one of the FMA,
+                // against the oracle's three), inside tol_x.
                 const double r = tv * inv[s];
                 const double q = floor(r);
                 const double fr = r - q;
-                const double ph = tv - per[s] * q;
-                const double x = ph * inv_bd;
+                const double x = fma(q, -per[s], tb);
                 // x - floor(x) in one instruction; for an unflagged x >= 0
                 // the conversion's truncation is that floor
                 const double fx = __builtin_amdgcn_fract(x);
@@ -382,16 +387,21 @@ __device__ __forceinline__ void bin_points(
             // registers in the main loop.
             double tx = tv;
             asm volatile("" : "+v"(tx));
+            // the periods themselves, read here rather than kept in
+            // registers through the loop (pg is opaque to hoisting)
+            const double *pgx = pg;
+            asm volatile("" : "+s"(pgx));
 #pragma unroll
             for (int s = 0; s < C; ++s)
                 if (fl[s]) {
+                    const double Ps = pgx[s];
                     const double r = tx * inv[s];
                     double q = floor(r);
                     const double fr = r - q;
                     const int a =
-                        base[s] + 1 + (int)((tx - per[s] * q) * inv_bd);
-                    if (fr <= tol_r || fr >= hr) q = floor(tx / per[s]);
-                    const double ph = tx - per[s] * q;
+                        base[s] + 1 + (int)fma(q, -per[s], tx * inv_bd);
+                    if (fr <= tol_r || fr >= hr) q = floor(tx / Ps);
+                    const double ph = tx - Ps * q;
                     const int e =
                         base[s] + min(max((int)trunc(ph / bd) + 1, 0), nb[s]);
                     if (e != a) {
@@ -647,7 +657,11 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     // quotient can change floor() only within 8 ulp of an integer, i.e.
     // within 2^-50 of the bound, so 2^-49 flags a superset of those points
     P.tol_r = ldexp((tmax - tmin) / pmin + 1.0, -49);
-    P.tol_x = ldexp(ceil(pmax / P.bd) + b.oversample + 2.0, -49);
+    // k_main's x = t'/bd - q*P/bd is within u*(5*t'/bd + 6*P/bd) of the
+    // oracle's quotient (u = 2^-53); 2^-49 * (t'/bd + P/bd + ...) covers it
+    P.tol_x = ldexp((tmax - tmin) / P.bd + ceil(pmax / P.bd) +
+                        b.oversample + 2.0,
+                    -49);
 
     // pass C: lane-major workspace (slot = step * NT + lane).  Points whose
     // first-cycle bin quotient t'/bd lies at a bin edge (exactly gridded
@@ -1412,7 +1426,8 @@ void k_main(Batch b, Out o, Work w,
                     const int nbs = (int)ceil(ps / bd) + os;
                     if (used + (nbs + 1) + (packed ? (nbs + 2) / 2 : 0) <=
                         cap) {
-                        per[s] = ps;
+                        // the exact fold takes P/bd (see bin_points)
+                        per[s] = b.mode == ATG_MODE_FAST ? ps : ps * inv_bd;
                         inv[s] = 1.0 / ps;
                         nb[s] = nbs;
                         base[s] = used;
@@ -1454,7 +1469,7 @@ void k_main(Batch b, Out o, Work w,
 #define ATG_BIN(C, E, K)                                                    \
     case C:                                                                 \
         bin_points<C, E, K>(tp, wp, yp, L, per, inv, nb, base, bd, inv_bd,  \
-                            P.tol_r, P.tol_x, binW, binY);                  \
+                            P.tol_r, P.tol_x, binW, binY, b.periods + p);   \
         break;
 #define ATG_BIN8(E, K)                                                      \
     switch (cnt) {                                                          \
