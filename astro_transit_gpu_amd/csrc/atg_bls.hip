@@ -77,7 +77,6 @@ struct LcParams {
     int n;            // points
     int status;       // ATG_LC_*
     int n_dur;
-    double tol_r;     // near-integer tolerance of t'/P (uniform bound)
     double tol_x;     // near-integer tolerance of phase/bd (uniform bound)
     double seq_y;     // parity mode: the oracle's totals, summed in FP64
     double seq_iv;    // in point order (sum of y*ivar, sum of ivar)
@@ -299,16 +298,16 @@ __device__ int lower_key(const double *tn, int n, double P, double invP,
 // constant so that the C fold chains are independent straight-line code the
 // scheduler can interleave; the next point's loads are issued before the
 // current point is folded.  The near-integer test uses the curve's uniform
-// tolerances (tol_r, tol_x: bounds of the per-point ones of fold_bin): an
-// unflagged point's reciprocal bin is the oracle's and lies in [1, nb - os];
-// a flagged point (a superset of the uncertain ones) is first added to its
-// reciprocal bin, within [1, nb], and then moved to the bin the oracle's
-// divisions give.
+// tolerance tol_x (a bound of the per-point ones of fold_bin, widened for the
+// shared quotient below): an unflagged point's reciprocal bin is the
+// oracle's and lies in [1, nb - os]; a flagged point (a superset of the
+// uncertain ones) is first added to its reciprocal bin, within [1, nb], and
+// then moved to the bin the oracle's divisions give.
 template <int C, bool EXACT, bool PACKED>
 __device__ __forceinline__ void bin_points(
     const double *tp, const long long *wp, const long long *yp, int L,
     const double *per, const double *inv, const int *nb, const int *base,
-    double bd, double inv_bd, double tol_r, double tol_x, long long *binW,
+    double bd, double inv_bd, double tol_x, long long *binW,
     long long *binY, const double *pg) {
     // tp, wp, yp: the curve's (uniform) slot bases; lane tid reads slots
     // tid + i * NT through a 32-bit offset, so that the loads address
@@ -316,7 +315,7 @@ __device__ __forceinline__ void bin_points(
     unsigned int sl = threadIdx.x;
     double tv_n = tp[sl];
     long long wv_n = wp[sl], yv_n = PACKED ? 0 : yp[sl];
-    const double hr = 1.0 - tol_r, hx = 1.0 - tol_x;
+    const double hx = 1.0 - tol_x;
     // per-period bin origins (bin 1 for the exact fold, bin 0 for the fast
     // one), so that a bin's LDS address is one shift-and-add of its index
     long long *ow[C], *oy[C];
@@ -348,16 +347,18 @@ __device__ __forceinline__ void bin_points(
                 // per[s] holds P/bd here: x = t'/bd - q*P/bd is within
                 // u*(5*t'/bd + 6*P/bd) of the oracle's (t' - P*q)/bd (u =
                 // 2^-53: three roundings of the products, one of the FMA,
-                // against the oracle's three), inside tol_x.
+                // against the oracle's three), inside tol_x.  The cycle
+                // q = floor(t' * (1/P)) is the oracle's floor(t'/P) or off
+                // by one where t'/P is within a few ulp of an integer; one
+                // too high puts x within tol_x below 0 (fract near 1), one
+                // too low within tol_x of P/bd: both are flagged.
                 const double r = tv * inv[s];
                 const double q = floor(r);
-                const double fr = r - q;
                 const double x = fma(q, -per[s], tb);
                 // x - floor(x) in one instruction; for an unflagged x >= 0
                 // the conversion's truncation is that floor
                 const double fx = __builtin_amdgcn_fract(x);
-                fl[s] = (fr <= tol_r) | (fr >= hr) | (fx <= tol_x) |
-                        (fx >= hx);
+                fl[s] = (fx <= tol_x) | (fx >= hx) | (x - per[s] >= -tol_x);
                 any |= fl[s];
                 at[s] = (int)x;  // bin at[s] + 1
             } else {
@@ -378,13 +379,11 @@ __device__ __forceinline__ void bin_points(
         if (EXACT && any) {
             // Every point went to its reciprocal bin above, so the atomics
             // share the fold's basic block.  A flagged point whose oracle
-            // bin differs moves there: integer adds commute, so the
-            // histogram is the one the oracle's bins give.  The cycle
-            // division runs only where r itself is within tol_r of an
-            // integer (otherwise floor(r) is the oracle's floor(t/P)).  tx
-            // hides the folds above from common-subexpression elimination:
-            // keeping their values live into this rare path would cost
-            // registers in the main loop.
+            // bin (both divisions) differs moves there: integer adds
+            // commute, so the histogram is the one the oracle's bins give.
+            // tx hides the folds above from common-subexpression
+            // elimination: keeping their values live into this rare path
+            // would cost registers in the main loop.
             double tx = tv;
             asm volatile("" : "+v"(tx));
             // the periods themselves, read here rather than kept in
@@ -395,13 +394,10 @@ __device__ __forceinline__ void bin_points(
             for (int s = 0; s < C; ++s)
                 if (fl[s]) {
                     const double Ps = pgx[s];
-                    const double r = tx * inv[s];
-                    double q = floor(r);
-                    const double fr = r - q;
-                    const int a =
-                        base[s] + 1 + (int)fma(q, -per[s], tx * inv_bd);
-                    if (fr <= tol_r || fr >= hr) q = floor(tx / Ps);
-                    const double ph = tx - Ps * q;
+                    const int a = base[s] + 1 +
+                                  (int)fma(floor(tx * inv[s]), -per[s],
+                                           tx * inv_bd);
+                    const double ph = tx - Ps * floor(tx / Ps);
                     const int e =
                         base[s] + min(max((int)trunc(ph / bd) + 1, 0), nb[s]);
                     if (e != a) {
@@ -653,12 +649,10 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     P.w_unit = ldexp(1.0, -sw);
     P.y_unit = ldexp(1.0, -sy);
     P.bd = dmin / (double)b.oversample;
-    // bounds of |t'/P| and |phase/bd| over the curve; a product-based
-    // quotient can change floor() only within 8 ulp of an integer, i.e.
-    // within 2^-50 of the bound, so 2^-49 flags a superset of those points
-    P.tol_r = ldexp((tmax - tmin) / pmin + 1.0, -49);
     // k_main's x = t'/bd - q*P/bd is within u*(5*t'/bd + 6*P/bd) of the
-    // oracle's quotient (u = 2^-53); 2^-49 * (t'/bd + P/bd + ...) covers it
+    // oracle's quotient (u = 2^-53), and a cycle off by one puts x within
+    // u*(8*t'/bd + 9*P/bd) of 0 or of P/bd; 2^-49 * (t'/bd + P/bd + ...)
+    // covers both
     P.tol_x = ldexp((tmax - tmin) / P.bd + ceil(pmax / P.bd) +
                         b.oversample + 2.0,
                     -49);
@@ -875,6 +869,12 @@ __device__ __forceinline__ void write_nan(const Out &o, long long p) {
     o.log_like[p] = nan;
 }
 
+// fmod(x, P) for x >= 0: x itself below P and x - P below 2P (exact by
+// Sterbenz's lemma, so equal to fmod), the library call beyond.
+__device__ __forceinline__ double fold_mod(double x, double P) {
+    return x < P ? x : (x < 2.0 * P ? x - P : fmod(x, P));
+}
+
 // Sliding-box scan of one period's prefix arrays (Cw or the point counts
 // Cn, and Cy; index 0..nb).  Duration-major: a thread meets its windows in
 // increasing index (k * nval + s), so a later equal score never replaces
@@ -1022,7 +1022,7 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
             const double snr = depth / derr;
             const double ll = 0.5 * win * depth * depth;
             const double dur = m * P.bd;
-            const double phase = fmod(s * P.bd + 0.5 * dur + 0.0, period);
+            const double phase = fold_mod(s * P.bd + 0.5 * dur + 0.0, period);
             o.power[p] = like ? ll : snr;
             o.depth[p] = depth;
             o.depth_err[p] = derr;
@@ -1310,7 +1310,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
             o.depth[p] = depth;
             o.depth_err[p] = derr;
             o.duration[p] = dur;
-            o.transit_time[p] = fmod(n * P.bd + 0.5 * dur + 0.0, period) +
+            o.transit_time[p] = fold_mod(n * P.bd + 0.5 * dur + 0.0, period) +
                                 P.t_ref;
             o.depth_snr[p] = snr;
             o.log_like[p] = ll;
@@ -1469,7 +1469,7 @@ void k_main(Batch b, Out o, Work w,
 #define ATG_BIN(C, E, K)                                                    \
     case C:                                                                 \
         bin_points<C, E, K>(tp, wp, yp, L, per, inv, nb, base, bd, inv_bd,  \
-                            P.tol_r, P.tol_x, binW, binY, b.periods + p);   \
+                            P.tol_x, binW, binY, b.periods + p);            \
         break;
 #define ATG_BIN8(E, K)                                                      \
     switch (cnt) {                                                          \
@@ -1964,7 +1964,7 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
         o.depth[p] = depth;
         o.depth_err[p] = derr;
         o.duration[p] = dur;
-        o.transit_time[p] = fmod(st * bd + 0.5 * dur + 0.0, period) + P.t_ref;
+        o.transit_time[p] = fold_mod(st * bd + 0.5 * dur + 0.0, period) + P.t_ref;
         o.depth_snr[p] = snr;
         o.log_like[p] = ll;
     }
