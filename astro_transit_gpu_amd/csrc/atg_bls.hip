@@ -324,6 +324,9 @@ __device__ __forceinline__ void bin_points(
         ow[s] = binW + base[s] + (EXACT ? 1 : 0);
         oy[s] = binY + base[s] + (EXACT ? 1 : 0);
     }
+    // two points per iteration: the prefetched registers alternate instead
+    // of being copied
+#pragma unroll 2
     for (int i = 0; i < L; ++i) {
         const double tv = tv_n;
         const long long wv = wv_n, yv = yv_n;
