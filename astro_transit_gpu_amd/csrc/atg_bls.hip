@@ -350,18 +350,18 @@ __device__ __forceinline__ void bin_points(
                 // per[s] holds P/bd here: x = t'/bd - q*P/bd is within
                 // u*(5*t'/bd + 6*P/bd) of the oracle's (t' - P*q)/bd (u =
                 // 2^-53: three roundings of the products, one of the FMA,
-                // against the oracle's three), inside tol_x.  The cycle
-                // q = floor(t' * (1/P)) is the oracle's floor(t'/P) or off
-                // by one where t'/P is within a few ulp of an integer; one
-                // too high puts x within tol_x below 0 (fract near 1), one
-                // too low within tol_x of P/bd: both are flagged.
-                const double r = tv * inv[s];
-                const double q = floor(r);
+                // against the oracle's three), inside tol_x.  inv[s] is
+                // 1/P scaled up by 1 + 2^-48, so t' * inv[s] exceeds the
+                // oracle's rounded t'/P and the cycle q is the oracle's or
+                // one too high, never too low; one too high (t'/P within
+                // ~36 ulp below an integer) puts x within tol_x below 0,
+                // i.e. fract(x) near 1: flagged.
+                const double q = floor(tv * inv[s]);
                 const double x = fma(q, -per[s], tb);
                 // x - floor(x) in one instruction; for an unflagged x >= 0
                 // the conversion's truncation is that floor
                 const double fx = __builtin_amdgcn_fract(x);
-                fl[s] = (fx <= tol_x) | (fx >= hx) | (x - per[s] >= -tol_x);
+                fl[s] = (fx <= tol_x) | (fx >= hx);
                 any |= fl[s];
                 at[s] = (int)x;  // bin at[s] + 1
             } else {
@@ -653,12 +653,12 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     P.y_unit = ldexp(1.0, -sy);
     P.bd = dmin / (double)b.oversample;
     // k_main's x = t'/bd - q*P/bd is within u*(5*t'/bd + 6*P/bd) of the
-    // oracle's quotient (u = 2^-53), and a cycle off by one puts x within
-    // u*(8*t'/bd + 9*P/bd) of 0 or of P/bd; 2^-49 * (t'/bd + P/bd + ...)
-    // covers both
+    // oracle's quotient (u = 2^-53), and a cycle one too high puts x within
+    // u*(40*t'/bd + 39*P/bd) below 0; 2^-47 * (t'/bd + P/bd + ...) covers
+    // both
     P.tol_x = ldexp((tmax - tmin) / P.bd + ceil(pmax / P.bd) +
                         b.oversample + 2.0,
-                    -49);
+                    -47);
 
     // pass C: lane-major workspace (slot = step * NT + lane).  Points whose
     // first-cycle bin quotient t'/bd lies at a bin edge (exactly gridded
@@ -1429,9 +1429,11 @@ void k_main(Batch b, Out o, Work w,
                     const int nbs = (int)ceil(ps / bd) + os;
                     if (used + (nbs + 1) + (packed ? (nbs + 2) / 2 : 0) <=
                         cap) {
-                        // the exact fold takes P/bd (see bin_points)
-                        per[s] = b.mode == ATG_MODE_FAST ? ps : ps * inv_bd;
-                        inv[s] = 1.0 / ps;
+                        // the exact fold takes P/bd and 1/P scaled up by
+                        // 1 + 2^-48 (see bin_points)
+                        const bool fastm = b.mode == ATG_MODE_FAST;
+                        per[s] = fastm ? ps : ps * inv_bd;
+                        inv[s] = fastm ? 1.0 / ps : (1.0 / ps) * (1.0 + 0x1p-48);
                         nb[s] = nbs;
                         base[s] = used;
                         used += nbs + 1;
