@@ -315,7 +315,7 @@ __device__ __forceinline__ void bin_points(
     unsigned int sl = threadIdx.x;
     double tv_n = tp[sl];
     long long wv_n = wp[sl], yv_n = PACKED ? 0 : yp[sl];
-    const double hx = 1.0 - tol_x;
+    const double tol2 = 2.0 * tol_x;
     // per-period bin origins (bin 1 for the exact fold, bin 0 for the fast
     // one), so that a bin's LDS address is one shift-and-add of its index
     long long *ow[C], *oy[C];
@@ -339,29 +339,31 @@ __device__ __forceinline__ void bin_points(
         int at[C];
         bool fl[C];
         bool any = false;
-        // t'/bd, shared by the C periods of the pass
-        const double tb = EXACT ? tv * inv_bd : 0.0;
+        // t'/bd + tol_x, shared by the C periods of the pass
+        const double tb = EXACT ? tv * inv_bd + tol_x : 0.0;
 #pragma unroll
         for (int s = 0; s < C; ++s) {
             if (EXACT) {
                 // bin from the rounded reciprocals; a quotient within the
                 // uniform tolerance of an integer is flagged for the
                 // oracle's divisions.  Unflagged bins lie in [1, nb - os].
-                // per[s] holds P/bd here: x = t'/bd - q*P/bd is within
-                // u*(5*t'/bd + 6*P/bd) of the oracle's (t' - P*q)/bd (u =
-                // 2^-53: three roundings of the products, one of the FMA,
-                // against the oracle's three), inside tol_x.  inv[s] is
-                // 1/P scaled up by 1 + 2^-48, so t' * inv[s] exceeds the
-                // oracle's rounded t'/P and the cycle q is the oracle's or
-                // one too high, never too low; one too high (t'/P within
-                // ~36 ulp below an integer) puts x within tol_x below 0,
-                // i.e. fract(x) near 1: flagged.
+                // per[s] holds P/bd here: x = t'/bd - q*P/bd + tol_x is
+                // within u*(7*t'/bd + 6*P/bd) of the oracle's
+                // (t' - P*q)/bd + tol_x (u = 2^-53: the roundings of the
+                // products, the shift and the FMA against the oracle's
+                // three), less than tol_x.  So an x whose fraction exceeds
+                // 2*tol_x lies in the oracle's integer interval, and the
+                // conversion's truncation is the oracle's floor; a fraction
+                // up to 2*tol_x is flagged.  inv[s] is 1/P scaled up by
+                // 1 + 2^-48, so t' * inv[s] exceeds the oracle's rounded
+                // t'/P and the cycle q is the oracle's or one too high; one
+                // too high (t'/P within ~36 ulp below an integer) puts the
+                // unshifted quotient within u*(40*t'/bd + 39*P/bd) below 0,
+                // so x within [0, 2*tol_x]: flagged.
                 const double q = floor(tv * inv[s]);
                 const double x = fma(q, -per[s], tb);
-                // x - floor(x) in one instruction; for an unflagged x >= 0
-                // the conversion's truncation is that floor
                 const double fx = __builtin_amdgcn_fract(x);
-                fl[s] = (fx <= tol_x) | (fx >= hx);
+                fl[s] = fx <= tol2;
                 any |= fl[s];
                 at[s] = (int)x;  // bin at[s] + 1
             } else {
@@ -399,7 +401,7 @@ __device__ __forceinline__ void bin_points(
                     const double Ps = pgx[s];
                     const int a = base[s] + 1 +
                                   (int)fma(floor(tx * inv[s]), -per[s],
-                                           tx * inv_bd);
+                                           tx * inv_bd + tol_x);
                     const double ph = tx - Ps * floor(tx / Ps);
                     const int e =
                         base[s] + min(max((int)trunc(ph / bd) + 1, 0), nb[s]);
