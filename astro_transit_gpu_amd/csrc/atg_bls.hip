@@ -861,6 +861,39 @@ struct Score {
             offer(oq, od, oi);
         }
     }
+    // One step of wave_reduce_last: combine with the lane that DPP control
+    // CTRL names (lanes outside ROW_MASK, or reading past their row, keep
+    // their own value, and combining a score with itself changes nothing).
+    template <int CTRL, int ROW_MASK>
+    __device__ __forceinline__ void dpp_step() {
+        const int ql = __builtin_amdgcn_update_dpp(
+            __double2loint(q), __double2loint(q), CTRL, ROW_MASK, 0xF, false);
+        const int qh = __builtin_amdgcn_update_dpp(
+            __double2hiint(q), __double2hiint(q), CTRL, ROW_MASK, 0xF, false);
+        const int dl = __builtin_amdgcn_update_dpp(__double2loint(den),
+                                                   __double2loint(den), CTRL,
+                                                   ROW_MASK, 0xF, false);
+        const int dh = __builtin_amdgcn_update_dpp(__double2hiint(den),
+                                                   __double2hiint(den), CTRL,
+                                                   ROW_MASK, 0xF, false);
+        const int oi =
+            __builtin_amdgcn_update_dpp(idx, idx, CTRL, ROW_MASK, 0xF, false);
+        offer(__hiloint2double(qh, ql), __hiloint2double(dh, dl), oi);
+    }
+    // The wave's best into lane 63 with DPP lane moves (no LDS traffic):
+    // quad xor 1 and 2, half-row and row mirrors (each row of 16 then
+    // holds its best), then rows 1 and 3 take rows 0 and 2 (row_bcast:15)
+    // and rows 2 and 3 take row 1 (row_bcast:31).  offer() is commutative
+    // and associative (highest score, then lowest index), so the order of
+    // the combination does not change the result.
+    __device__ __forceinline__ void wave_reduce_last() {
+        dpp_step<0xB1, 0xF>();   // quad_perm [1,0,3,2]
+        dpp_step<0x4E, 0xF>();   // quad_perm [2,3,0,1]
+        dpp_step<0x141, 0xF>();  // row_half_mirror
+        dpp_step<0x140, 0xF>();  // row_mirror
+        dpp_step<0x142, 0xA>();  // row_bcast:15
+        dpp_step<0x143, 0xC>();  // row_bcast:31
+    }
 };
 
 __device__ __forceinline__ void write_nan(const Out &o, long long p) {
@@ -993,16 +1026,19 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
                                    epsn);
     }
     // workgroup argmax (highest score, then lowest (k, n))
-    best.wave_reduce();
+    best.wave_reduce_last();
     const int wv = tid >> 6, l = tid & 63;
     __syncthreads();
-    if (l == 0) {
+    if (l == 63) {
         dscr[wv] = best.q;
         dscr[NW + wv] = best.den;
         iscr[wv] = best.idx;
     }
     __syncthreads();
     if (tid == 0) {
+        best.q = dscr[0];
+        best.den = dscr[NW];
+        best.idx = iscr[0];
         for (int i = 1; i < NW; ++i) best.offer(dscr[i], dscr[NW + i], iscr[i]);
         const int bi = best.idx;
         if (bi == INT_MAX) {  // no window qualifies: power -inf as astropy
