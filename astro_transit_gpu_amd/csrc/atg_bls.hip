@@ -201,6 +201,47 @@ __device__ T block_excl_scan(T v, T *scratch, T *total) {
     return off + inc - v;
 }
 
+// One step of the int64 DPP scan below: v + (the value CTRL names), 0 for
+// lanes outside ROW_MASK or reading past their row.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ long long dpp_add_i64(long long v) {
+    const int lo = (int)(unsigned long long)v;
+    const int hi = (int)((unsigned long long)v >> 32);
+    const unsigned int l2 =
+        (unsigned int)__builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, 0xF, true);
+    const unsigned int h2 =
+        (unsigned int)__builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xF, true);
+    return v + (long long)(((unsigned long long)h2 << 32) | l2);
+}
+
+// Exclusive int64 scan over the workgroup in thread order, the wave part
+// with DPP lane moves (row shifts 1, 2, 4, 8, then row broadcasts 15 and
+// 31: an inclusive scan without LDS traffic); *total gets the sum.
+__device__ __forceinline__ long long block_excl_scan_dpp(long long v,
+                                                         long long *scratch,
+                                                         long long *total) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    long long inc = v;
+    inc = dpp_add_i64<0x111, 0xF>(inc);  // row_shr:1
+    inc = dpp_add_i64<0x112, 0xF>(inc);  // row_shr:2
+    inc = dpp_add_i64<0x114, 0xF>(inc);  // row_shr:4
+    inc = dpp_add_i64<0x118, 0xF>(inc);  // row_shr:8
+    inc = dpp_add_i64<0x142, 0xA>(inc);  // row_bcast:15
+    inc = dpp_add_i64<0x143, 0xC>(inc);  // row_bcast:31
+    __syncthreads();
+    if (l == 63) scratch[w] = inc;
+    __syncthreads();
+    long long off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        const long long t = scratch[i];
+        if (i < w) off += t;
+        tot += t;
+    }
+    *total = tot;
+    return off + inc - v;
+}
+
 struct MinOp {
     __device__ double operator()(double a, double b) const { return fmin(a, b); }
 };
@@ -981,8 +1022,8 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
         if (!packed) sy += binY[base + i];
     }
     long long totw, toty;
-    long long ow = block_excl_scan(sw, lscr, &totw);
-    long long oy = packed ? 0 : block_excl_scan(sy, lscr, &toty);
+    long long ow = block_excl_scan_dpp(sw, lscr, &totw);
+    long long oy = packed ? 0 : block_excl_scan_dpp(sy, lscr, &toty);
     // cumulative sums in units of the total weight (FP64).  Packed bins:
     // the flux prefix replaces the histogram in place and the weight prefix
     // is kept as the int32 point count Cn just above it (12 bytes a bin);
@@ -1254,8 +1295,8 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
             sy += binY[i];
         }
         long long tw, ty;
-        long long ow = block_excl_scan(sw, lscr, &tw) + cw;
-        long long oy = block_excl_scan(sy, lscr, &ty) + cy;
+        long long ow = block_excl_scan_dpp(sw, lscr, &tw) + cw;
+        long long oy = block_excl_scan_dpp(sy, lscr, &ty) + cy;
         for (int i = b0; i < b1; ++i) {
             ow += binW[i];
             oy += binY[i];
