@@ -313,6 +313,20 @@ __device__ __forceinline__ int fold_bin_in(double t, double P, double q,
     return min(max((int)xi + 1, 0), nb);
 }
 
+// Point-order copies are stored in blocks of NT points: within a block,
+// point 4*l + v (l = 0..63, v = 0..NW-1) sits in slot 64*v + l, so that the
+// thread tid = 64*v + l of a workgroup walking a block reads slot tid (one
+// contiguous 512-byte line set per wave) while the points of a wave's lanes
+// are NW apart in time and mostly fall in distinct bins.  Each curve's copy
+// starts at lc_off[c] + NT*c and is padded to whole blocks.
+static_assert(NT == 256 && NW == 4, "natural-copy blocks assume 4 wave64");
+__device__ __forceinline__ int nat_slot(int i) {
+    return (i & ~(NT - 1)) | ((i & (NW - 1)) << 6) | ((i >> 2) & 63);
+}
+__device__ __forceinline__ long long nat_base(const Batch &b, int lc) {
+    return b.lc_off[lc] + (long long)NT * lc;
+}
+
 // For time-sorted points the key (cycle, bin) of the oracle's fold is
 // non-decreasing in the point index: floor(t/P) is monotone in t and, for a
 // fixed cycle, so are t - P*q, the division by bd and the truncation.  The
@@ -324,7 +338,7 @@ __device__ int lower_key(const double *tn, int n, double P, double invP,
     int lo = 0, hi = n;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        const double t = tn[mid];
+        const double t = tn[nat_slot(mid)];
         const double q = fold_cycle(t, P, invP, fast);
         const bool less =
             q < c ||
@@ -785,12 +799,19 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
                            nbmax + 1 > (double)min(lds_bins, EXACT_BINS);
     if (P.sorted && (big || big_exact)) {
         P.natural = 1;
-        for (int i = tid; i < n; i += NT) {
-            long long wq, yq;
-            quantize(b, off + i, med, wscale, yscale, wq, yq);
-            w.nat_t[off + i] = b.t[off + i] - tmin;
-            w.nat_w[off + i] = wq;
-            w.nat_y[off + i] = yq;
+        const long long nbase = nat_base(b, lc);
+        const int nfull = (n + NT - 1) & ~(NT - 1);
+        for (int i = tid; i < nfull; i += NT) {
+            long long wq = 0, yq = 0;
+            double tv = 0.0;
+            if (i < n) {
+                quantize(b, off + i, med, wscale, yscale, wq, yq);
+                tv = b.t[off + i] - tmin;
+            }
+            const long long sl = nbase + nat_slot(i);
+            w.nat_t[sl] = tv;
+            w.nat_w[sl] = wq;
+            w.nat_y[sl] = yq;
         }
     }
     P.Wq_tot = wq_sum;
@@ -1143,6 +1164,50 @@ __device__ __forceinline__ void seg_add(int j, int s0, int hi, long long wv,
     }
 }
 
+// Ranged fold of one cycle's point range [a, e) of a time-sorted curve
+// into the LDS segment of bins [s0, hi], one block of the natural copy per
+// step: thread tid takes slot tid, which holds point nat_slot^-1(tid) of the
+// block.  PAD selects the destination: the point's own bin r (unless the
+// padding assignment overwrites it), or, for r in 1..os, the padding copy
+// r + nb - os - 1.  Points outside [a, e) and points of zero weight and
+// flux are masked at the atomic, so that four blocks' loads are issued
+// before the first point is folded; every loaded block starts below e and
+// so lies inside the curve's padded copy.
+template <bool FAST, bool PAD>
+__device__ __forceinline__ void big_fold_range(
+    const double *tn, const long long *wn, const long long *yn, int a, int e,
+    double period, double qc, double bd, double inv_bd, int nb, int os,
+    int s0, int hi, long long *binW, long long *binY) {
+    const int tid = threadIdx.x;
+    const int pt = (tid & 63) * NW + (tid >> 6);  // point of slot tid
+    auto add = [&](int j, double t, long long wv, long long yv) {
+        const int r = fold_bin_in(t, period, qc, bd, inv_bd, nb, FAST);
+        const int d = PAD ? r + nb - os - 1 : r;
+        const bool own = PAD ? (r >= 1 && r <= os) : (r < nb - os || r == nb);
+        if (own && d >= s0 && d <= hi && j >= a && j < e && (wv | yv) != 0) {
+            __hip_atomic_fetch_add(&binW[d - s0], wv, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(&binY[d - s0], yv, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    };
+    int B = a & ~(NT - 1);
+    for (; B + 3 * NT < e; B += 4 * NT) {
+        double t[4];
+        long long wv[4], yv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            t[u] = tn[B + u * NT + tid];
+            wv[u] = wn[B + u * NT + tid];
+            yv[u] = yn[B + u * NT + tid];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) add(B + u * NT + pt, t[u], wv[u], yv[u]);
+    }
+    for (; B < e; B += NT)
+        add(B + pt, tn[B + tid], wn[B + tid], yn[B + tid]);
+}
+
 // One period whose n_bins + 1 exceeds the LDS histogram: the start
 // positions are processed in segments of S start positions; segment k keeps
 // bins [s0, s0 + S + mmax] in LDS on top of the exact int64 sum of all bins
@@ -1174,18 +1239,18 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
     const double fw = P.w_unit / Wt, fy = P.y_unit / Wt;
     const double c1 = (double)P.wq_one * (P.w_unit / P.W_tot);
     // ranged form: cycles 0..Q of the sorted points
-    const double *tn = w.nat_t + b.lc_off[P.lc_self];
+    const long long nbase = nat_base(b, P.lc_self);
+    const double *tn = w.nat_t + nbase;
     int ncyc = 0;
     if (P.natural)
-        ncyc = (int)fold_cycle(tn[P.n - 1], period, invP, fastm) + 1;
+        ncyc = (int)fold_cycle(tn[nat_slot(P.n - 1)], period, invP, fastm) + 1;
     const bool ranged = P.natural && ncyc <= (1 << 20) &&
                         lds_bins - ncyc - mmax - 1 >= 64;
     const int S = ranged ? lds_bins - ncyc - mmax - 1 : lds_bins - mmax - 1;
     int *rng = (int *)(binW + (lds_bins - ncyc));  // [2c], [2c+1]
     int *erng = (int *)(binY + (lds_bins - ncyc));
-    const long long *wn = w.nat_w + b.lc_off[P.lc_self];
-    const long long *yn = w.nat_y + b.lc_off[P.lc_self];
-    const int perm = (tid & 63) * NW + (tid >> 6);
+    const long long *wn = w.nat_w + nbase;
+    const long long *yn = w.nat_y + nbase;
     Score best;
     double bwin = 0.0, byin = 0.0;
     const double *tp = w.ws_t + P.ws_off + tid;
@@ -1224,45 +1289,30 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
         }
         __syncthreads();
         if (ranged) {
-            // direct destinations: bins [s0, hi] of every cycle
+            // direct destinations: bins [s0, hi] of every cycle; then the
+            // padding copies, bins 1..os of every cycle to nb-os..nb-1
             for (int c = 0; c < ncyc; ++c) {
-                const int e = rng[2 * c + 1];
-                const double qc = (double)c;
-                for (int j = rng[2 * c] + perm; j < e; j += NT) {
-                    const long long wv = wn[j], yv = yn[j];
-                    if ((wv | yv) == 0) continue;
-                    const int r = fold_bin_in(tn[j], period, qc, bd, inv_bd,
-                                              nb, fastm);
-                    if ((r < nb - os || r == nb) && r >= s0 && r <= hi) {
-                        __hip_atomic_fetch_add(&binW[r - s0], wv,
-                                               __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                        __hip_atomic_fetch_add(&binY[r - s0], yv,
-                                               __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                    }
-                }
+                const int j0 = rng[2 * c], e = rng[2 * c + 1];
+                if (fastm)
+                    big_fold_range<true, false>(tn, wn, yn, j0, e, period,
+                                                (double)c, bd, inv_bd, nb, os,
+                                                s0, hi, binW, binY);
+                else
+                    big_fold_range<false, false>(tn, wn, yn, j0, e, period,
+                                                 (double)c, bd, inv_bd, nb,
+                                                 os, s0, hi, binW, binY);
             }
-            // padding copies: bins 1..os of every cycle to nb-os..nb-1
             if (extra) {
                 for (int c = 0; c < ncyc; ++c) {
-                    const int e = erng[2 * c + 1];
-                    const double qc = (double)c;
-                    for (int j = erng[2 * c] + perm; j < e; j += NT) {
-                        const long long wv = wn[j], yv = yn[j];
-                        if ((wv | yv) == 0) continue;
-                        const int r = fold_bin_in(tn[j], period, qc, bd,
-                                                  inv_bd, nb, fastm);
-                        const int d = r + nb - os - 1;
-                        if (r >= 1 && r <= os && d >= s0 && d <= hi) {
-                            __hip_atomic_fetch_add(
-                                &binW[d - s0], wv, __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_WORKGROUP);
-                            __hip_atomic_fetch_add(
-                                &binY[d - s0], yv, __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_WORKGROUP);
-                        }
-                    }
+                    const int j0 = erng[2 * c], e = erng[2 * c + 1];
+                    if (fastm)
+                        big_fold_range<true, true>(tn, wn, yn, j0, e, period,
+                                                   (double)c, bd, inv_bd, nb,
+                                                   os, s0, hi, binW, binY);
+                    else
+                        big_fold_range<false, true>(tn, wn, yn, j0, e, period,
+                                                    (double)c, bd, inv_bd, nb,
+                                                    os, s0, hi, binW, binY);
                 }
             }
             __syncthreads();
@@ -1777,10 +1827,10 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
     // of bins 1..os whose copies land in the segment.  Each bin still
     // receives its points in point order.
     const bool fits = nb + 1 <= lds_bins;
-    const double *tn = w.nat_t + off;
+    const double *tn = w.nat_t + nat_base(b, lc);
     int ncyc = 0;
     if (!fits && P.natural)
-        ncyc = (int)fold_cycle(tn[n - 1], period, invP) + 1;
+        ncyc = (int)fold_cycle(tn[nat_slot(n - 1)], period, invP) + 1;
     const int T = 2 * ncyc + 2;  // tail slots of my[] for the range table
     const bool ranged = !fits && P.natural && nb - os > os &&
                         ncyc <= (1 << 20) && lds_bins - T - mmax - 1 >= 64;
@@ -2691,7 +2741,7 @@ Layout layout_of(const atg_bls_batch *b) {
     L.t = o;       o = align256(o + slots * sizeof(double));
     L.w = o;       o = align256(o + slots * sizeof(long long));
     L.y = o;       o = align256(o + slots * sizeof(long long));
-    const size_t npt = (size_t)b->n_points;
+    const size_t npt = (size_t)b->n_points + nlc * NT;  // whole blocks
     L.nt = o;      o = align256(o + npt * sizeof(double));
     L.nw = o;      o = align256(o + npt * sizeof(long long));
     L.ny = o;      o = align256(o + npt * sizeof(long long));
