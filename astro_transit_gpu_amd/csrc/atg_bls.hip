@@ -1169,43 +1169,81 @@ __device__ __forceinline__ void seg_add(int j, int s0, int hi, long long wv,
 // step: thread tid takes slot tid, which holds point nat_slot^-1(tid) of the
 // block.  PAD selects the destination: the point's own bin r (unless the
 // padding assignment overwrites it), or, for r in 1..os, the padding copy
-// r + nb - os - 1.  Points outside [a, e) and points of zero weight and
-// flux are masked at the atomic, so that four blocks' loads are issued
-// before the first point is folded; every loaded block starts below e and
+// r + nb - os - 1.  Points outside [a, e) are masked at the atomic, so
+// that four blocks' loads are issued before the first point is folded
+// (points of zero weight add zero); every loaded block starts below e and
 // so lies inside the curve's padded copy.
+// A workgroup-uniform global pointer moved to scalar registers, and loads
+// from it at a 32-bit byte offset (the base + offset addressing of global
+// loads, with no 64-bit address per lane).
+typedef __attribute__((address_space(1))) const char gchar;
+__device__ __forceinline__ gchar *scalar_base(const void *p) {
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (gchar *)(((unsigned long long)hi << 32) | lo);
+}
+template <typename T>
+__device__ __forceinline__ T load_at(gchar *base, unsigned byte_off) {
+    return *(const __attribute__((address_space(1))) T *)(base + byte_off);
+}
+
+// (the curve's padded copy must span less than 2^32 bytes)
 template <bool FAST, bool PAD>
 __device__ __forceinline__ void big_fold_range(
     const double *tn, const long long *wn, const long long *yn, int a, int e,
     double period, double qc, double bd, double inv_bd, int nb, int os,
     int s0, int hi, long long *binW, long long *binY) {
-    const int tid = threadIdx.x;
+    const unsigned tid = threadIdx.x;
     const int pt = (tid & 63) * NW + (tid >> 6);  // point of slot tid
+    gchar *tb = scalar_base(tn), *wb = scalar_base(wn), *yb = scalar_base(yn);
+    const unsigned span = (unsigned)(hi - s0), cnt = (unsigned)(e - a);
+    // exact fold with one uniform tolerance: x = ph/bd lies in [-1, nb + 1]
+    // and the reciprocal product (with the shift) is within
+    // (nb + 2) * 2^-51 of it, so a point whose shifted product xs keeps
+    // more than 2*tol above its integer part has trunc(xs) == trunc(ph/bd);
+    // the others (a 2*tol share of them) take the division of fold_bin_in
+    const double tol = ldexp((double)nb + 2.0, -48), pq = period * qc;
+    auto bin_of = [&](double t) -> int {
+        if (FAST) return fold_bin_in(t, period, qc, bd, inv_bd, nb, true);
+        const double ph = t - pq;
+        const double xs = fma(ph, inv_bd, tol);
+        const double xi = trunc(xs);
+        if (xs - xi <= 2.0 * tol)
+            return min(max((int)trunc(ph / bd) + 1, 0), nb);
+        return min((int)xi + 1, nb);
+    };
     auto add = [&](int j, double t, long long wv, long long yv) {
-        const int r = fold_bin_in(t, period, qc, bd, inv_bd, nb, FAST);
+        const int r = bin_of(t);
         const int d = PAD ? r + nb - os - 1 : r;
         const bool own = PAD ? (r >= 1 && r <= os) : (r < nb - os || r == nb);
-        if (own && d >= s0 && d <= hi && j >= a && j < e && (wv | yv) != 0) {
-            __hip_atomic_fetch_add(&binW[d - s0], wv, __ATOMIC_RELAXED,
+        const unsigned l = (unsigned)(d - s0);
+        if (own & (l <= span) & ((unsigned)(j - a) < cnt)) {
+            __hip_atomic_fetch_add(&binW[l], wv, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(&binY[d - s0], yv, __ATOMIC_RELAXED,
+            __hip_atomic_fetch_add(&binY[l], yv, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     };
     int B = a & ~(NT - 1);
     for (; B + 3 * NT < e; B += 4 * NT) {
+        const unsigned o = ((unsigned)B + tid) * 8u;
         double t[4];
         long long wv[4], yv[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            t[u] = tn[B + u * NT + tid];
-            wv[u] = wn[B + u * NT + tid];
-            yv[u] = yn[B + u * NT + tid];
+            t[u] = load_at<double>(tb, o + u * NT * 8);
+            wv[u] = load_at<long long>(wb, o + u * NT * 8);
+            yv[u] = load_at<long long>(yb, o + u * NT * 8);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) add(B + u * NT + pt, t[u], wv[u], yv[u]);
     }
-    for (; B < e; B += NT)
-        add(B + pt, tn[B + tid], wn[B + tid], yn[B + tid]);
+    for (; B < e; B += NT) {
+        const unsigned o = ((unsigned)B + tid) * 8u;
+        add(B + pt, load_at<double>(tb, o), load_at<long long>(wb, o),
+            load_at<long long>(yb, o));
+    }
 }
 
 // One period whose n_bins + 1 exceeds the LDS histogram: the start
@@ -1245,6 +1283,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
     if (P.natural)
         ncyc = (int)fold_cycle(tn[nat_slot(P.n - 1)], period, invP, fastm) + 1;
     const bool ranged = P.natural && ncyc <= (1 << 20) &&
+                        P.n < (1 << 28) &&
                         lds_bins - ncyc - mmax - 1 >= 64;
     const int S = ranged ? lds_bins - ncyc - mmax - 1 : lds_bins - mmax - 1;
     int *rng = (int *)(binW + (lds_bins - ncyc));  // [2c], [2c+1]
