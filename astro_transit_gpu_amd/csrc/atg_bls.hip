@@ -566,7 +566,10 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     const long long n64 = b.lc_off[lc + 1] - off;
     const long long po = b.per_off[lc], np = b.per_off[lc + 1] - po;
     const long long dof = b.dur_off[lc], nd = b.dur_off[lc + 1] - dof;
-    const int n = (int)n64;
+    // point indices within a curve are 32-bit (with headroom for the
+    // padded per-curve slots)
+    const bool too_long = n64 > ((long long)1 << 30);
+    const int n = too_long ? 0 : (int)n64;
     const int tid = threadIdx.x;
     if (tid == 0) w.lc_flags[lc] = 0;  // workspace bytes are reused across shapes
     LcParams P;
@@ -577,7 +580,8 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     P.ws_off = off + (long long)lc * (2 * NT);
     P.L = (n + NT - 1) / NT;
     int status = ATG_LC_OK;
-    if (n <= 0 || np <= 0 || nd <= 0) status = ATG_LC_EMPTY;
+    if (too_long) status = ATG_LC_TOO_MANY_POINTS;
+    else if (n <= 0 || np <= 0 || nd <= 0) status = ATG_LC_EMPTY;
     else if (nd > DMAX) status = ATG_LC_TOO_MANY_DURATIONS;
 
     // grid checks in the oracle's order: periods, then durations

@@ -51,6 +51,7 @@ extern "C" {
 #define ATG_LC_TOO_MANY_BINS 5 /* a period's bins exceed the LDS budget    */
 #define ATG_LC_EMPTY 6        /* no points, periods or durations           */
 #define ATG_LC_TOO_MANY_DURATIONS 7 /* more than 64 durations             */
+#define ATG_LC_TOO_MANY_POINTS 8 /* more than 2^30 points in one curve     */
 
 /* element types */
 #define ATG_F32 1

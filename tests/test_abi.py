@@ -35,6 +35,15 @@ def test_abi_version(lib):
     assert lib.atg_abi_version() == 1
 
 
+def test_curve_status_codes_match_header():
+    """Every per-curve status the header defines has its message in the
+    Python table the API raises from, and no other code does."""
+    from astro_transit_gpu_amd import _abi
+    src = open(os.path.join(ROOT, "include", "atg_bls.h")).read()
+    codes = {int(v) for v in re.findall(r"#define ATG_LC_[A-Z_]+ (\d+)", src)}
+    assert codes == set(_abi.LC_STATUS), (codes, sorted(_abi.LC_STATUS))
+
+
 def test_struct_layout_matches_header():
     """ctypes structs have the C sizes (8-byte aligned pointers)."""
     from astro_transit_gpu_amd import _abi
