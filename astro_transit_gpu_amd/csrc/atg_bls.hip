@@ -608,6 +608,11 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     if (status == ATG_LC_OK) {
         if (bad_grid & 1) status = ATG_LC_BAD_PERIOD;
         else if ((bad_grid & 2) || dmax > pmin) status = ATG_LC_BAD_DURATION;
+        // bin indices are 32-bit: n_bins = ceil(P / bd) + oversample with
+        // bd = min(durations) / oversample must stay within 2^30
+        else if (!(ceil(pmax / (dmin / (double)b.oversample)) + b.oversample <=
+                   (double)(1 << 30)))
+            status = ATG_LC_TOO_MANY_BINS;
     }
 
     // pass A: min(t), NaN and dy checks, time order

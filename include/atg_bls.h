@@ -48,7 +48,8 @@ extern "C" {
                                  < DBL_EPSILON (astropy code 2)            */
 #define ATG_LC_NAN 3          /* NaN in t or y                             */
 #define ATG_LC_BAD_DY 4       /* dy <= 0 or NaN                            */
-#define ATG_LC_TOO_MANY_BINS 5 /* a period's bins exceed the LDS budget    */
+#define ATG_LC_TOO_MANY_BINS 5 /* a period's bins exceed the LDS budget
+                                  (or 2^30 bins)                          */
 #define ATG_LC_EMPTY 6        /* no points, periods or durations           */
 #define ATG_LC_TOO_MANY_DURATIONS 7 /* more than 64 durations             */
 #define ATG_LC_TOO_MANY_POINTS 8 /* more than 2^30 points in one curve     */

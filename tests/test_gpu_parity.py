@@ -134,6 +134,26 @@ def test_status_codes(ctx):
     assert np.isfinite(h["power"][:20]).all()
 
 
+@pytest.mark.parametrize("mode", ["parity", "fast"])
+def test_bin_count_limit(ctx, mode):
+    """A grid whose bins would overflow 32-bit indices (a 1e-9 d shortest
+    duration: 5e10 bins at P = 5 d) is rejected as too many bins, with NaN
+    results, while the other curve of the batch runs normally."""
+    from astro_transit_gpu_amd import DeviceBatch
+    rng = np.random.default_rng(5)
+    t = np.sort(rng.uniform(0, 20, 500))
+    y = 1 + 1e-3 * rng.standard_normal(500)
+    dy = np.full(500, 1e-3)
+    per = np.linspace(1, 5, 20)
+    b = DeviceBatch.from_host([t, t], [y, y], [dy, dy], [per, per],
+                              [np.array([0.1, 0.2]), np.array([1e-9, 0.1])],
+                              device=ctx.device)
+    h = ctx.run(b, objective="likelihood", mode=mode).to_host()
+    assert list(h["status"]) == [0, 5]
+    assert np.isfinite(h["power"][:20]).all()
+    assert np.isnan(h["power"][20:]).all()
+
+
 @pytest.mark.parametrize("name", ["cfg1_small_likelihood", "cfg1_small_snr",
                                   "hetero_dy", "unsorted_t"])
 def test_segmented_bins_bit_identical(name):
