@@ -498,18 +498,31 @@ __device__ void select2(const void *y, int dt, long long off, int n, int k0,
         for (int i = threadIdx.x; i < 512; i += NT) hist[i] = 0;
         __syncthreads();
         unsigned c0 = 0, c1 = 0, d0 = 0, d1 = 0;
-        for (int i = threadIdx.x; i < n; i += NT) {
-            unsigned long long key = okey(load_val(y, dt, off + i));
-            unsigned d = (unsigned)(key >> shift) & 255u;
-            if ((key & mask) == pre0) {
-                if (c0 && d != d0) { atomicAdd(&hist[d0], c0); c0 = 0; }
-                d0 = d;
-                ++c0;
-            }
-            if ((key & mask) == pre1) {
-                if (c1 && d != d1) { atomicAdd(&hist[256 + d1], c1); c1 = 0; }
-                d1 = d;
-                ++c1;
+        // four values per step, loaded before the first is counted (the
+        // thread's order of points is unchanged)
+        for (int i0 = threadIdx.x; i0 < n; i0 += 4 * NT) {
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                v[u] = load_val(y, dt, off + min(i0 + u * NT, n - 1));
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (i0 + u * NT >= n) break;
+                unsigned long long key = okey(v[u]);
+                unsigned d = (unsigned)(key >> shift) & 255u;
+                if ((key & mask) == pre0) {
+                    if (c0 && d != d0) { atomicAdd(&hist[d0], c0); c0 = 0; }
+                    d0 = d;
+                    ++c0;
+                }
+                if ((key & mask) == pre1) {
+                    if (c1 && d != d1) {
+                        atomicAdd(&hist[256 + d1], c1);
+                        c1 = 0;
+                    }
+                    d1 = d;
+                    ++c1;
+                }
             }
         }
         if (c0) atomicAdd(&hist[d0], c0);
@@ -538,23 +551,38 @@ __device__ void select2(const void *y, int dt, long long off, int n, int k0,
     *v1 = ikey(pre1);
 }
 
+// Raw y and dy of point i (dy = 1 when the curve has none).
+__device__ __forceinline__ void load_point(const Batch &b, long long i,
+                                           double &y, double &d) {
+    y = load_val(b.y, b.y_dt, i);
+    d = b.dy ? load_val(b.dy, b.dy_dt, i) : 1.0;
+}
+
+// Fixed-point weight and weight*flux of a point from its raw y and dy.
+__device__ __forceinline__ void quantize_raw(const Batch &b, double y,
+                                             double d, double med,
+                                             double wscale, double yscale,
+                                             long long &wq, long long &yq) {
+    double iv = 1.0;
+    if (b.dy) iv = 1.0 / (d * d);
+    const double yc = y - med;
+    wq = llrint(iv * wscale);
+    yq = llrint((yc * iv) * yscale);
+}
+
 // Fixed-point weight and weight*flux of point i (shared by both layouts).
 __device__ __forceinline__ void quantize(const Batch &b, long long i,
                                          double med, double wscale,
                                          double yscale, long long &wq,
                                          long long &yq) {
-    double iv = 1.0;
-    if (b.dy) {
-        const double d = load_val(b.dy, b.dy_dt, i);
-        iv = 1.0 / (d * d);
-    }
-    const double yc = load_val(b.y, b.y_dt, i) - med;
-    wq = llrint(iv * wscale);
-    yq = llrint((yc * iv) * yscale);
+    double y, d;
+    load_point(b, i, y, d);
+    quantize_raw(b, y, d, med, wscale, yscale, wq, yq);
 }
 
 __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     __shared__ unsigned int hist[512];
+    __shared__ double seq_v[8 * NT];
     __shared__ int sel[4];
     __shared__ unsigned int uscr[NW];
     __shared__ double dscr[NW];
@@ -618,16 +646,25 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     // pass A: min(t), NaN and dy checks, time order
     double tmin = INFINITY, tmax = -INFINITY;
     int bad = 0;
-    for (int i = tid; i < n; i += NT) {
-        double tv = b.t[off + i];
-        tmax = fmax(tmax, tv);
-        double yv = load_val(b.y, b.y_dt, off + i);
-        if (isnan(tv) || isnan(yv)) bad |= 1;
-        if (i > 0 && !(tv >= b.t[off + i - 1])) bad |= 4;
-        tmin = fmin(tmin, tv);
-        if (b.dy) {
-            double d = load_val(b.dy, b.dy_dt, off + i);
-            if (!(d > 0.0)) bad |= 2;
+    // four points per step, loaded first (indices past the end re-read
+    // the last point, which changes no minimum, maximum or flag)
+    for (int i0 = tid; i0 < n; i0 += 4 * NT) {
+        double tv[4], tp[4], yv[4], dv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = min(i0 + u * NT, n - 1);
+            tv[u] = b.t[off + i];
+            tp[u] = b.t[off + max(i - 1, 0)];
+            load_point(b, off + i, yv[u], dv[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = min(i0 + u * NT, n - 1);
+            tmax = fmax(tmax, tv[u]);
+            tmin = fmin(tmin, tv[u]);
+            if (isnan(tv[u]) || isnan(yv[u])) bad |= 1;
+            if (i > 0 && !(tv[u] >= tp[u])) bad |= 4;
+            if (b.dy && !(dv[u] > 0.0)) bad |= 2;
         }
     }
     tmin = block_reduce(tmin, MinOp(), dscr);
@@ -654,23 +691,33 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
 
     // parity mode: the oracle's FP64 totals in its order (one wave walks
     // the points; the refinement kernel uses them)
-    if (b.mode == ATG_MODE_PARITY && tid < 64) {
+    // The workgroup stages chunks of 4*NT values in LDS (four loads per
+    // thread in flight) and the first wave adds them up in point order from
+    // broadcast reads.
+    if (b.mode == ATG_MODE_PARITY) {
         double sy = 0.0, si = 0.0;
-        for (int base = 0; base < n; base += 64) {
-            const int j = base + tid;
-            double yv = 0.0, iv = 1.0;
-            if (j < n) {
-                if (b.dy) {
-                    const double d = load_val(b.dy, b.dy_dt, off + j);
-                    iv = 1.0 / (d * d);
+        for (int base = 0; base < n; base += 4 * NT) {
+            double yr[4], dr[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                load_point(b, off + min(base + u * NT + tid, n - 1), yr[u],
+                           dr[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double iv = b.dy ? 1.0 / (dr[u] * dr[u]) : 1.0;
+                seq_v[u * NT + tid] = (yr[u] - med) * iv;
+                seq_v[4 * NT + u * NT + tid] = iv;
+            }
+            __syncthreads();
+            if (tid < 64) {
+                const int cnt = min(4 * NT, n - base);
+#pragma unroll 8
+                for (int k = 0; k < cnt; ++k) {
+                    sy = sy + seq_v[k];
+                    si = si + seq_v[4 * NT + k];
                 }
-                yv = (load_val(b.y, b.y_dt, off + j) - med) * iv;
             }
-            const int cnt = min(64, n - base);
-            for (int k = 0; k < cnt; ++k) {
-                sy = sy + readlane_d(yv, k);
-                si = si + readlane_d(iv, k);
-            }
+            __syncthreads();
         }
         P.seq_y = sy;
         P.seq_iv = si;
@@ -678,17 +725,22 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
 
     // pass B: magnitudes for the fixed-point scales, weight uniformity
     double sw_sum = 0.0, sy_sum = 0.0, iv_min = INFINITY, iv_max = -INFINITY;
-    for (int i = tid; i < n; i += NT) {
-        double iv = 1.0;
-        if (b.dy) {
-            double d = load_val(b.dy, b.dy_dt, off + i);
-            iv = 1.0 / (d * d);
+    for (int i0 = tid; i0 < n; i0 += 4 * NT) {
+        double yr[4], dr[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            load_point(b, off + min(i0 + u * NT, n - 1), yr[u], dr[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (i0 + u * NT >= n) break;
+            double iv = 1.0;
+            if (b.dy) iv = 1.0 / (dr[u] * dr[u]);
+            double yc = yr[u] - med;
+            sw_sum += iv;
+            sy_sum += fabs(yc * iv);
+            iv_min = fmin(iv_min, iv);
+            iv_max = fmax(iv_max, iv);
         }
-        double yc = load_val(b.y, b.y_dt, off + i) - med;
-        sw_sum += iv;
-        sy_sum += fabs(yc * iv);
-        iv_min = fmin(iv_min, iv);
-        iv_max = fmax(iv_max, iv);
     }
     sw_sum = block_reduce(sw_sum, AddOp<double>(), dscr);
     sy_sum = block_reduce(sy_sum, AddOp<double>(), dscr);
@@ -742,9 +794,16 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     };
     const int Lr = P.L;  // read chunk: thread tid reads [tid*Lr, (tid+1)*Lr)
     int my_edges = 0;
-    for (int i = 0; i < Lr; ++i) {
-        const int src = tid * Lr + i;
-        if (src < n && is_edge(b.t[off + src] - tmin)) ++my_edges;
+    for (int i0 = 0; i0 < Lr; i0 += 4) {
+        double tr[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            tr[u] = b.t[off + min(tid * Lr + i0 + u, n - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int src = tid * Lr + i0 + u;
+            if (i0 + u < Lr && src < n && is_edge(tr[u] - tmin)) ++my_edges;
+        }
     }
     int n_edge;
     const int edge_before = block_excl_scan(my_edges, iscr, &n_edge);
@@ -770,27 +829,37 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
             w.ws_y[slot] = yq_pad;
         }
     int e_seen = 0;
-    for (int i = 0; i < Lr; ++i) {
-        const int src = tid * Lr + i;
-        if (src >= n) break;
-        const double tp = b.t[off + src] - tmin;
-        long long wq, yq;
-        quantize(b, off + src, med, wscale, yscale, wq, yq);
-        long long slot;
-        if (is_edge(tp)) {
-            const int e = edge_before + e_seen++;
-            slot = P.ws_off + (long long)(L0 + e / NT) * NT + e % NT;
-        } else {
-            const int d = src - edge_before - e_seen;  // rank among the rest
-            slot = P.ws_off + (long long)(d % L0) * NT + d / L0;
+    for (int i0 = 0; i0 < Lr; i0 += 4) {
+        double tr[4], yr[4], dr[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int src = min(tid * Lr + i0 + u, n - 1);
+            tr[u] = b.t[off + src];
+            load_point(b, off + src, yr[u], dr[u]);
         }
-        w.ws_t[slot] = tp;
-        // packed curves keep the packed bin increment in ws_w
-        w.ws_w[slot] = P.packed ? (1ll << P.ybits) + yq : wq;
-        w.ws_y[slot] = yq;
-        if (src == 0) P.wq_one = wq;
-        wq_sum += wq;
-        yq_sum += yq;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int src = tid * Lr + i0 + u;
+            if (i0 + u >= Lr || src >= n) break;
+            const double tp = tr[u] - tmin;
+            long long wq, yq;
+            quantize_raw(b, yr[u], dr[u], med, wscale, yscale, wq, yq);
+            long long slot;
+            if (is_edge(tp)) {
+                const int e = edge_before + e_seen++;
+                slot = P.ws_off + (long long)(L0 + e / NT) * NT + e % NT;
+            } else {
+                const int d = src - edge_before - e_seen;  // rank among the rest
+                slot = P.ws_off + (long long)(d % L0) * NT + d / L0;
+            }
+            w.ws_t[slot] = tp;
+            // packed curves keep the packed bin increment in ws_w
+            w.ws_w[slot] = P.packed ? (1ll << P.ybits) + yq : wq;
+            w.ws_y[slot] = yq;
+            if (src == 0) P.wq_one = wq;
+            wq_sum += wq;
+            yq_sum += yq;
+        }
     }
     wq_sum = block_reduce(wq_sum, AddOp<long long>(), lscr);
     yq_sum = block_reduce(yq_sum, AddOp<long long>(), lscr);
@@ -810,17 +879,30 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
         P.natural = 1;
         const long long nbase = nat_base(b, lc);
         const int nfull = (n + NT - 1) & ~(NT - 1);
-        for (int i = tid; i < nfull; i += NT) {
-            long long wq = 0, yq = 0;
-            double tv = 0.0;
-            if (i < n) {
-                quantize(b, off + i, med, wscale, yscale, wq, yq);
-                tv = b.t[off + i] - tmin;
+        for (int i0 = tid; i0 < nfull; i0 += 4 * NT) {
+            double tr[4], yr[4], dr[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = min(i0 + u * NT, n - 1);
+                tr[u] = b.t[off + i];
+                load_point(b, off + i, yr[u], dr[u]);
             }
-            const long long sl = nbase + nat_slot(i);
-            w.nat_t[sl] = tv;
-            w.nat_w[sl] = wq;
-            w.nat_y[sl] = yq;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = i0 + u * NT;
+                if (i >= nfull) break;
+                long long wq = 0, yq = 0;
+                double tv = 0.0;
+                if (i < n) {
+                    quantize_raw(b, yr[u], dr[u], med, wscale, yscale, wq,
+                                 yq);
+                    tv = tr[u] - tmin;
+                }
+                const long long sl = nbase + nat_slot(i);
+                w.nat_t[sl] = tv;
+                w.nat_w[sl] = wq;
+                w.nat_y[sl] = yq;
+            }
         }
     }
     P.Wq_tot = wq_sum;
