@@ -1921,6 +1921,71 @@ __global__ __launch_bounds__(NT) void k_cands(Batch b, Out o, Work w) {
     }
 }
 
+// One wave's chunk of points added into the LDS bins my/mi so that every
+// bin receives its points in chunk order (the oracle's point order): lane j
+// (j < cnt) adds (yv, iv) to bin dst (-1: to none).  (cur, ay, ai) is the
+// bin whose running sums the item-by-item walk keeps in registers (cur -1:
+// none).  When the chunk's destinations never decrease (time-sorted points
+// within one cycle), each bin's points are one run of lanes, preceded only
+// by the points that go nowhere: the first lane of every run adds the run's
+// values one after another from LDS, all runs at once.  Any other chunk is
+// walked point by point.  The sums are the same either way, add for add.
+__device__ __forceinline__ void exact_add_chunk(int dst, double yv, double iv,
+                                                int cnt, double *my,
+                                                double *mi, double *st_y,
+                                                double *st_i, int &cur,
+                                                double &ay, double &ai) {
+    const int lane = threadIdx.x & 63;
+    const bool act = lane < cnt;
+    const int prev = __shfl_up(dst, 1);
+    const bool down = act && lane > 0 && dst < prev;
+    if (__ballot(down) == 0) {
+        if (cur >= 0) {
+            if (lane == 0) {
+                my[cur] = ay;
+                mi[cur] = ai;
+            }
+            cur = -1;
+        }
+        st_y[lane] = yv;
+        st_i[lane] = iv;
+        __builtin_amdgcn_wave_barrier();
+        const unsigned long long starts =
+            __ballot(act && (lane == 0 || dst != prev));
+        if (act && dst >= 0 && (lane == 0 || dst != prev)) {
+            const unsigned long long later =
+                lane < 63 ? starts >> (lane + 1) : 0ull;
+            const int end = later ? lane + 1 + __builtin_ctzll(later) : cnt;
+            double a = my[dst], c = mi[dst];
+            for (int k = lane; k < end; ++k) {
+                a += st_y[k];
+                c += st_i[k];
+            }
+            my[dst] = a;
+            mi[dst] = c;
+        }
+        __builtin_amdgcn_wave_barrier();
+        return;
+    }
+    for (int j = 0; j < cnt; ++j) {
+        const int tj = __builtin_amdgcn_readlane(dst, j);
+        if (tj < 0) continue;
+        const double yvj = readlane_d(yv, j);
+        const double ivj = readlane_d(iv, j);
+        if (tj != cur) {
+            if (cur >= 0 && lane == 0) {
+                my[cur] = ay;
+                mi[cur] = ai;
+            }
+            cur = tj;
+            ay = my[tj];
+            ai = mi[tj];
+        }
+        ay += yvj;
+        ai += ivj;
+    }
+}
+
 __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
                                               int lds_bins) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1928,6 +1993,7 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
     double *mi = my + lds_bins;         // sum of ivar per bin
     __shared__ int s_m[DMAX];
     __shared__ double s_tot[2];
+    __shared__ double st_y[64], st_i[64];  // a chunk's values, staged
     __shared__ unsigned int s_q;
     const int lane = threadIdx.x;
     for (;;) {
@@ -2057,24 +2123,8 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
                     y_raw = load_val(b.y, b.y_dt, off + nj);
                     if (b.dy) d_raw = load_val(b.dy, b.dy_dt, off + nj);
                 }
-                const int cnt = min(64, total - cb);
-                for (int jj = 0; jj < cnt; ++jj) {
-                    const int tj = __builtin_amdgcn_readlane(loc, jj);
-                    if (tj < 0) continue;
-                    const double yvj = readlane_d(yv, jj);
-                    const double ivj = readlane_d(iv, jj);
-                    if (tj != cur) {
-                        if (cur >= 0 && lane == 0) {
-                            my[cur] = ay;
-                            mi[cur] = ai;
-                        }
-                        cur = tj;
-                        ay = my[tj];
-                        ai = mi[tj];
-                    }
-                    ay += yvj;
-                    ai += ivj;
-                }
+                exact_add_chunk(loc, yv, iv, min(64, total - cb), my, mi,
+                                st_y, st_i, cur, ay, ai);
             }
         } else {
         __syncthreads();
@@ -2098,44 +2148,33 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
                 y_raw = load_val(b.y, b.y_dt, off + nx);
                 if (b.dy) d_raw = load_val(b.dy, b.dy_dt, off + nx);
             }
+            const int cnt = min(64, n - base);
+            if (fits) {
+                exact_add_chunk(ind, yv, iv, cnt, my, mi, st_y, st_i, cur,
+                                ay, ai);
+                continue;
+            }
             // the wave walks the chunk in point order; every lane holds the
             // same running values, lane 0 stores them
-            const int cnt = min(64, n - base);
             for (int j = 0; j < cnt; ++j) {
                 const double yvj = readlane_d(yv, j), ivj = readlane_d(iv, j);
                 const int indj = __builtin_amdgcn_readlane(ind, j);
-                if (fits) {
-                    // the bin's running sum stays in registers while
-                    // consecutive points share it: same order of adds
-                    if (indj != cur) {
-                        if (cur >= 0 && lane == 0) {
-                            my[cur] = ay;
-                            mi[cur] = ai;
-                        }
-                        cur = indj;
-                        ay = my[indj];
-                        ai = mi[indj];
+                int j1, j2;
+                pad_targets(indj, nb, os, j1, j2);
+                if (j1 >= s0 && j1 <= hi) {
+                    const double a = my[j1 - s0] + yvj;
+                    const double bb = mi[j1 - s0] + ivj;
+                    if (lane == 0) {
+                        my[j1 - s0] = a;
+                        mi[j1 - s0] = bb;
                     }
-                    ay += yvj;
-                    ai += ivj;
-                } else {
-                    int j1, j2;
-                    pad_targets(indj, nb, os, j1, j2);
-                    if (j1 >= s0 && j1 <= hi) {
-                        const double a = my[j1 - s0] + yvj;
-                        const double bb = mi[j1 - s0] + ivj;
-                        if (lane == 0) {
-                            my[j1 - s0] = a;
-                            mi[j1 - s0] = bb;
-                        }
-                    }
-                    if (j2 >= s0 && j2 <= hi) {
-                        const double a = my[j2 - s0] + yvj;
-                        const double bb = mi[j2 - s0] + ivj;
-                        if (lane == 0) {
-                            my[j2 - s0] = a;
-                            mi[j2 - s0] = bb;
-                        }
+                }
+                if (j2 >= s0 && j2 <= hi) {
+                    const double a = my[j2 - s0] + yvj;
+                    const double bb = mi[j2 - s0] + ivj;
+                    if (lane == 0) {
+                        my[j2 - s0] = a;
+                        mi[j2 - s0] = bb;
                     }
                 }
             }
