@@ -2196,7 +2196,24 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
             double ci = s0 == 0 ? mi[0] : cprev_i + mi[0];
             my[0] = cy;
             mi[0] = ci;
-            for (int l = 1; l < len; ++l) {
+            // eight bins read ahead of their adds (the adds stay in order)
+            int l = 1;
+            for (; l + 8 <= len; l += 8) {
+                double vy[8], vi[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    vy[u] = my[l + u];
+                    vi[u] = mi[l + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    cy = cy + vy[u];
+                    ci = ci + vi[u];
+                    my[l + u] = cy;
+                    mi[l + u] = ci;
+                }
+            }
+            for (; l < len; ++l) {
                 cy = cy + my[l];
                 ci = ci + mi[l];
                 my[l] = cy;
