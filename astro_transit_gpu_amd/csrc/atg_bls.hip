@@ -1203,27 +1203,15 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
             o.depth_snr[p] = 0.0;
             o.log_like[p] = 0.0;
         } else {
+            // the winner's window sums; k_fields derives the seven fields
+            // from them in parallel over periods (the duration slot carries
+            // -(index + 1) until then)
             const int k = bi / nval, s = bi - k * nval, m = s_m[k];
-            const double win =
+            o.depth[p] =
                 (packed ? (double)(Cn[s + m] - Cn[s]) * c1 : Cw[s + m] - Cw[s]) *
                 Wt;
-            const double yin_s = (Cy[s + m] - Cy[s]) * Wt;
-            const double wout = Wt - win;
-            const double y_in = yin_s / win;
-            const double y_out = (Yt - yin_s) / wout;
-            const double depth = y_out - y_in;
-            const double derr = sqrt(1.0 / win + 1.0 / wout);
-            const double snr = depth / derr;
-            const double ll = 0.5 * win * depth * depth;
-            const double dur = m * P.bd;
-            const double phase = fold_mod(s * P.bd + 0.5 * dur + 0.0, period);
-            o.power[p] = like ? ll : snr;
-            o.depth[p] = depth;
-            o.depth_err[p] = derr;
-            o.duration[p] = dur;
-            o.transit_time[p] = phase + P.t_ref;
-            o.depth_snr[p] = snr;
-            o.log_like[p] = ll;
+            o.depth_err[p] = (Cy[s + m] - Cy[s]) * Wt;
+            o.duration[p] = -(double)bi - 1.0;
         }
     }
     __syncthreads();
@@ -1829,6 +1817,53 @@ void k_big(Batch b, Out o, Work w, int lds_bins) {
             write_nan(o, p);
             atomicOr(&w.lc_flags[lc], 1);
         }
+    }
+}
+
+// ---------------------------------------------------------------- k_fields
+//
+// The seven result fields of the periods whose winner k_main found: k_main
+// leaves the winner's window weight W_in and flux sum in depth and
+// depth_err and -(index + 1) in duration, and this pass evaluates the
+// oracle's formulas for them in parallel over periods (astropy
+// timeseries/periodograms/bls/methods.py:110-139, core.py:719).  Periods
+// finished elsewhere (the phase tiles of k_big, no qualifying window,
+// rejected curves) hold a duration >= 0 or NaN and are left alone.
+__global__ __launch_bounds__(NT) void k_fields(Batch b, Out o, Work w) {
+    const int lc = blockIdx.x;
+    const LcParams P = w.prm[lc];
+    if (P.status != ATG_LC_OK) return;
+    const long long po = b.per_off[lc], np = b.per_off[lc + 1] - po;
+    const long long dof = b.dur_off[lc];
+    const bool like = b.objective == ATG_OBJ_LIKELIHOOD;
+    const double Wt = P.W_tot, Yt = P.Y_tot;
+    for (long long i = threadIdx.x; i < np; i += NT) {
+        const long long p = po + i;
+        const double mark = o.duration[p];
+        if (!(mark < 0.0)) continue;
+        const int bi = (int)(-mark - 1.0);
+        const double period = b.periods[p];
+        const int nval = (int)ceil(period / P.bd) + b.oversample + 1;
+        const int k = bi / nval, s = bi - k * nval;
+        const int m =
+            (short)fmin(round(b.durs[dof + k] / P.bd), 32767.0);  // as k_main
+        const double win = o.depth[p], yin_s = o.depth_err[p];
+        const double wout = Wt - win;
+        const double y_in = yin_s / win;
+        const double y_out = (Yt - yin_s) / wout;
+        const double depth = y_out - y_in;
+        const double derr = sqrt(1.0 / win + 1.0 / wout);
+        const double snr = depth / derr;
+        const double ll = 0.5 * win * depth * depth;
+        const double dur = m * P.bd;
+        const double phase = fold_mod(s * P.bd + 0.5 * dur + 0.0, period);
+        o.power[p] = like ? ll : snr;
+        o.depth[p] = depth;
+        o.depth_err[p] = derr;
+        o.duration[p] = dur;
+        o.transit_time[p] = phase + P.t_ref;
+        o.depth_snr[p] = snr;
+        o.log_like[p] = ll;
     }
 }
 
@@ -3129,7 +3164,8 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
             : c->tune.lds_bins;
     hipLaunchKernelGGL(k_big, dim3(grid), dim3(NT), (size_t)big_bins * 16,
                        c->stream, kb, ko, w, big_bins);
-    if (c->timing) {  // the periodogram stage: k_main and k_big
+    hipLaunchKernelGGL(k_fields, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w);
+    if (c->timing) {  // the periodogram stage: k_main, k_big and k_fields
         hipEventRecord(ev1, c->stream);
         c->pending.emplace_back(ev0, ev1);
     }

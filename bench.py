@@ -55,9 +55,9 @@ def algorithmic_work(n_points, periods, durations, oversample=10):
 
 
 def measured_traffic(n_lc):
-    """HBM bytes per launch of the periodogram stage (k_main + k_big) from
-    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel
-    calibrated), scaled to n_lc curves."""
+    """HBM bytes per launch of the periodogram stage (k_main, k_big and
+    k_fields) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (copy-kernel calibrated), scaled to n_lc curves."""
     path = os.path.join(ROOT, "profiles",
                         "k_main_k_big_hbm_traffic_1000x15k_5000p_5d.md")
     try:
@@ -466,7 +466,7 @@ def main():
             "traffic": measured_traffic(args.n_lc),
             # the periodogram stage: k_main and the phase-tile kernel k_big
             # that follows it (timed together with HIP events)
-            "kernel": "k_main+k_big", "kernel_ms": round(kms, 3),
+            "kernel": "k_main+k_big+k_fields", "kernel_ms": round(kms, 3),
             "valu_lane_ops_per_launch": per_launch_valu,
             "lds": {"achieved_GBps": round(ach_lds / 1e9, 1),
                     "peak_GBps": PEAK_LDS / 1e9,

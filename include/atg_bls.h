@@ -260,8 +260,9 @@ int atg_bls_model(atg_ctx *ctx, int64_t n, const double *t, double period,
                   double duration, double transit_time, double y_in,
                   double y_out, uint8_t *mask, double *y_model);
 
-/* Average duration (ms) of the periodogram stage (the main kernel and the
- * phase-tile kernel that follows it) over the runs since the last reset,
+/* Average duration (ms) of the periodogram stage (the main kernel, the
+ * phase-tile kernel and the result-field pass that follow it) over the runs
+ * since the last reset,
  * timed with HIP events on the context stream; resets when `reset` != 0.
  * Returns the number of timed runs (0 = none). */
 int atg_timing_main_kernel(atg_ctx *ctx, int enable, int reset,
