@@ -59,7 +59,7 @@ def measured_traffic(n_lc):
     k_fields) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
     (copy-kernel calibrated), scaled to n_lc curves."""
     path = os.path.join(ROOT, "profiles",
-                        "k_main_k_big_hbm_traffic_1000x15k_5000p_5d.md")
+                        "periodogram_stage_hbm_traffic_1000x15k_5000p_5d.md")
     try:
         kv = dict(line.strip().split("=", 1) for line in open(path)
                   if "=" in line and line.strip().split("=", 1)[0] in
