@@ -472,12 +472,15 @@ def main():
                     "peak_GBps": PEAK_LDS / 1e9,
                     "frac": round(ach_lds / PEAK_LDS, 4),
                     "bytes_per_launch": per_launch_lds},
-            # the survey's minimum (inputs 16 B/point, outputs) and what
-            # the stage reads once by design: the packed lane-major points
+            # the algorithmic minimum (inputs 16 B/point, outputs) and what
+            # the stage moves once by design: the packed lane-major points
             # (16 B/point; every period of this workload fits the LDS
-            # histogram, so no point-order reads), outputs
+            # histogram, so no point-order reads), each period's winner
+            # written by k_main and read back with its period by k_fields
+            # (24 + 32 B), outputs
             "hbm_algorithmic_bytes_per_launch": hbm_in + hbm_out,
-            "hbm_design_bytes_per_launch": args.n_lc * 15000 * 16 + hbm_out,
+            "hbm_design_bytes_per_launch": args.n_lc * 15000 * 16 + hbm_out +
+            args.n_lc * 5000 * 56,
         },
     }
     if not args.skip_extras:
