@@ -1610,8 +1610,14 @@ void k_main(Batch b, Out o, Work w,
             int qq = queue, left = NQ;
             unsigned int it = 0xffffffffu;
             while (left > 0) {
-                it = atomicAdd(&w.qnext[qq], 1u);
-                if ((long long)it < w.qtot[qq]) break;
+                // an exhausted queue is passed over with a load, not an
+                // atomic: at the end every workgroup visits all of them
+                if ((long long)__hip_atomic_load(&w.qnext[qq], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT) <
+                    w.qtot[qq]) {
+                    it = atomicAdd(&w.qnext[qq], 1u);
+                    if ((long long)it < w.qtot[qq]) break;
+                }
                 it = 0xffffffffu;
                 qq = (qq + 1) % NQ;
                 --left;
@@ -1789,7 +1795,15 @@ void k_big(Batch b, Out o, Work w, int lds_bins) {
     int q = blockIdx.x % NQ, empty = 0;
     while (empty < NQ) {
         __syncthreads();
-        if (tid == 0) s_j = atomicAdd(&w.big_next[q], 1u);
+        if (tid == 0) {
+            // an exhausted list (all of them, mostly) costs a load, not an
+            // atomic on one of eight counters shared by every workgroup
+            const unsigned int cnt = w.big_count[q];
+            s_j = __hip_atomic_load(&w.big_next[q], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT) < cnt
+                      ? atomicAdd(&w.big_next[q], 1u)
+                      : cnt;
+        }
         __syncthreads();
         const unsigned int j = s_j;
         if (j >= w.big_count[q]) {  // list q is done (k_main has ended)
@@ -2033,7 +2047,14 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
     const int lane = threadIdx.x;
     for (;;) {
     __syncthreads();
-    if (lane == 0) s_q = atomicAdd(w.cand_next, 1u);
+    if (lane == 0) {
+        // the drained queue costs a load, not an atomic on its one counter
+        const unsigned int cnt = *w.cand_count;
+        s_q = __hip_atomic_load(w.cand_next, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT) < cnt
+                  ? atomicAdd(w.cand_next, 1u)
+                  : cnt;
+    }
     __syncthreads();
     const unsigned int q = s_q;
     if (q >= *w.cand_count) break;
