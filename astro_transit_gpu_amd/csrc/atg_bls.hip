@@ -784,9 +784,12 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     // by the near-integer test of every period whose first cycle holds
     // them, which sends their whole wave through the divisions.  They go to
     // the last E_steps steps, lanes side by side, so that they share a few
-    // wave steps; every other point keeps the contiguous runs (lane j owns
-    // its slice of [j*L0, (j+1)*L0)).  The atomics are order-independent,
-    // so the placement changes nothing but the time.
+    // wave steps.  The other points fill the steps in time order, 256 per
+    // step, lane l of wave v taking point 4*l + v of the step: the lanes of
+    // a wave hold points 4 apart in time, which for every period fall in
+    // distinct, nearby bins (few LDS bank conflicts among the atomics; the
+    // phase wraps at most once within a step).  The atomics are
+    // order-independent, so the placement changes nothing but the time.
     const double inv_bd0 = 1.0 / P.bd, edge_tol = 4.0 * P.tol_x;
     auto is_edge = [&](double tp) {
         const double f = __builtin_amdgcn_fract(tp * inv_bd0);
@@ -815,7 +818,7 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     const long long yq_pad = 0;
     // padding slots of this lane: zero weight, an in-range time
     for (int i = 0; i < L0; ++i)
-        if ((long long)tid * L0 + i >= n0) {
+        if (i * NT + (tid & 63) * 4 + (tid >> 6) >= n0) {
             const long long slot = P.ws_off + (long long)i * NT + tid;
             w.ws_t[slot] = t_pad;
             w.ws_w[slot] = 0;
@@ -850,7 +853,9 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
                 slot = P.ws_off + (long long)(L0 + e / NT) * NT + e % NT;
             } else {
                 const int d = src - edge_before - e_seen;  // rank among the rest
-                slot = P.ws_off + (long long)(d % L0) * NT + d / L0;
+                const int q = d & (NT - 1);
+                slot = P.ws_off + (long long)(d >> 8) * NT + ((q & 3) << 6) +
+                       (q >> 2);
             }
             w.ws_t[slot] = tp;
             // packed curves keep the packed bin increment in ws_w
@@ -1610,11 +1615,13 @@ void k_main(Batch b, Out o, Work w,
             int qq = queue, left = NQ;
             unsigned int it = 0xffffffffu;
             while (left > 0) {
-                // an exhausted queue is passed over with a load, not an
-                // atomic: at the end every workgroup visits all of them
-                if ((long long)__hip_atomic_load(&w.qnext[qq], __ATOMIC_RELAXED,
+                // the current queue is tried at once; a queue to steal from
+                // is passed over with a load, not an atomic, once it is
+                // exhausted (at the end every workgroup visits all of them)
+                if (left == NQ ||
+                    (long long)__hip_atomic_load(&w.qnext[qq], __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT) <
-                    w.qtot[qq]) {
+                        w.qtot[qq]) {
                     it = atomicAdd(&w.qnext[qq], 1u);
                     if ((long long)it < w.qtot[qq]) break;
                 }
