@@ -2944,7 +2944,7 @@ struct atg_ctx {
     std::string err;
     void *ws = nullptr;
     size_t ws_bytes = 0;
-    atg_tuning tune{16, 2540, 4, 0};
+    atg_tuning tune{0, 2540, 4, 0};  // periods per item: per batch
     int num_cu = 0;
     size_t lds_max = 65536;
     size_t lds_attr_set = 65536;
@@ -3152,6 +3152,16 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
 
     const int nlc = (int)b->n_lc;
     const size_t lds = (size_t)c->tune.lds_bins * 16;
+    const int grid = c->num_cu * c->tune.wg_per_cu;
+    // periods per work item, unless the tiling fixes it: long items (fewer
+    // passes over each curve's points) when the batch has periods to spare
+    // for eight items per workgroup, short ones for a small batch, so that
+    // its few curves still spread over the whole GPU
+    const int per_item =
+        c->tune.periods_per_item > 0
+            ? c->tune.periods_per_item
+            : (int)std::min<long long>(
+                  24, std::max<long long>(4, b->n_periods / (8LL * grid)));
     if (lds > c->lds_attr_set) {
         for (const void *f : {(const void *)k_main, (const void *)k_big}) {
             e = hipFuncSetAttribute(
@@ -3163,8 +3173,7 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
     hipLaunchKernelGGL(k_prep, dim3(nlc), dim3(NT), 0, c->stream, kb, w,
                        c->tune.lds_bins);
     hipLaunchKernelGGL(k_items, dim3(1), dim3(NT), 0, c->stream,
-                       (long long)b->n_lc, kb.per_off,
-                       c->tune.periods_per_item, w);
+                       (long long)b->n_lc, kb.per_off, per_item, w);
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (c->timing) {
         for (int i = 0; i < 2; ++i) {
@@ -3179,9 +3188,8 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
         }
         hipEventRecord(ev0, c->stream);
     }
-    const int grid = c->num_cu * c->tune.wg_per_cu;
     hipLaunchKernelGGL(k_main, dim3(grid), dim3(NT), lds, c->stream, kb, ko,
-                       w, c->tune.periods_per_item, c->tune.lds_bins);
+                       w, per_item, c->tune.lds_bins);
     // k_big's histogram is 32 bins (512 B) smaller where that is what its
     // static LDS needs for as many workgroups per CU as k_main (its results
     // do not depend on the tile size)
@@ -3267,7 +3275,7 @@ int atg_ctx_set_tuning(atg_ctx *c, const atg_tuning *t) {
     if (t->periods_per_item) n.periods_per_item = t->periods_per_item;
     if (t->lds_bins) n.lds_bins = t->lds_bins;
     if (t->wg_per_cu) n.wg_per_cu = t->wg_per_cu;
-    if (n.periods_per_item < 1 || n.periods_per_item > 64)
+    if (n.periods_per_item < 0 || n.periods_per_item > 64)
         return fail(c, ATG_ERR_ARG, "periods_per_item must be in [1, 64]");
     if (n.lds_bins < 64 || (size_t)n.lds_bins * 16 > c->lds_max)
         return fail(c, ATG_ERR_ARG,

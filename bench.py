@@ -414,6 +414,9 @@ def main():
     t0 = time.perf_counter()
     if not args.autotune:
         tuning = ctx.get_tuning()
+        if tuning["periods_per_item"] == 0:  # chosen per batch
+            tuning["periods_per_item"] = (
+                "per batch: min(24, max(4, periods / (8 x workgroups)))")
     else:
         tuning, _ = autotune(ctx, d["t"], d["y"], d["dy"], d["periods"],
                              d["durations"], objective=objective,

@@ -116,7 +116,9 @@ typedef struct atg_bls_out {
 /* Kernel tiling (chosen by the autotuner; 0 fields keep the current value).
  */
 typedef struct atg_tuning {
-    int32_t periods_per_item; /* periods per work item (1..64)             */
+    int32_t periods_per_item; /* periods per work item (1..64; a context
+                                 starts at 0: min(24, max(4, periods /
+                                 (8 * workgroups))) per batch)            */
     int32_t lds_bins;         /* LDS histogram: lds_bins * 16 bytes        */
     int32_t wg_per_cu;        /* persistent workgroups per CU              */
     int32_t reserved;
