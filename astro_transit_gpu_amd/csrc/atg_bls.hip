@@ -795,21 +795,34 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
         const double f = __builtin_amdgcn_fract(tp * inv_bd0);
         return f <= edge_tol || f >= 1.0 - edge_tol;
     };
-    const int Lr = P.L;  // read chunk: thread tid reads [tid*Lr, (tid+1)*Lr)
-    int my_edges = 0;
-    for (int i0 = 0; i0 < Lr; i0 += 4) {
+    // Each wave walks a contiguous quarter of the points, 64 at a time, so
+    // that the rank of a point among the edge points (or among the rest) is
+    // a ballot count plus the wave's running carry, and the 64 points of a
+    // step land in four runs of 16 slots.
+    const int wv = tid >> 6, ln = tid & 63;
+    const int qlen = (n + NW - 1) / NW;
+    const int q0 = min(wv * qlen, n), q1 = min(q0 + qlen, n);
+    const unsigned long long below = (1ull << ln) - 1ull;
+    int my_edges = 0;  // edge points of this wave's quarter (every lane)
+    for (int i0 = q0; i0 < q1; i0 += 4 * 64) {
         double tr[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            tr[u] = b.t[off + min(tid * Lr + i0 + u, n - 1)];
+            tr[u] = b.t[off + min(i0 + u * 64 + ln, n - 1)];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int src = tid * Lr + i0 + u;
-            if (i0 + u < Lr && src < n && is_edge(tr[u] - tmin)) ++my_edges;
-        }
+        for (int u = 0; u < 4; ++u)
+            my_edges += __popcll(
+                __ballot(i0 + u * 64 + ln < q1 && is_edge(tr[u] - tmin)));
     }
-    int n_edge;
-    const int edge_before = block_excl_scan(my_edges, iscr, &n_edge);
+    __syncthreads();
+    if (ln == 0) iscr[wv] = my_edges;
+    __syncthreads();
+    int edge_before = 0, n_edge = 0;
+    for (int k = 0; k < NW; ++k) {
+        if (k < wv) edge_before += iscr[k];
+        n_edge += iscr[k];
+    }
+    __syncthreads();
     const int n0 = n - n_edge;
     const int L0 = (n0 + NT - 1) / NT, L1 = (n_edge + NT - 1) / NT;
     P.L = L0 + L1;
@@ -831,28 +844,32 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
             w.ws_w[slot] = 0;
             w.ws_y[slot] = yq_pad;
         }
-    int e_seen = 0;
-    for (int i0 = 0; i0 < Lr; i0 += 4) {
+    int carry = edge_before;  // edge points before the wave's next step
+    for (int i0 = q0; i0 < q1; i0 += 4 * 64) {
         double tr[4], yr[4], dr[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int src = min(tid * Lr + i0 + u, n - 1);
+            const int src = min(i0 + u * 64 + ln, n - 1);
             tr[u] = b.t[off + src];
             load_point(b, off + src, yr[u], dr[u]);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int src = tid * Lr + i0 + u;
-            if (i0 + u >= Lr || src >= n) break;
+            const int src = i0 + u * 64 + ln;
+            const bool valid = src < q1;
             const double tp = tr[u] - tmin;
+            const bool edge = valid && is_edge(tp);
+            const unsigned long long m = __ballot(edge);
+            const int e = carry + __popcll(m & below);  // edges before src
+            carry += __popcll(m);
+            if (!valid) continue;
             long long wq, yq;
             quantize_raw(b, yr[u], dr[u], med, wscale, yscale, wq, yq);
             long long slot;
-            if (is_edge(tp)) {
-                const int e = edge_before + e_seen++;
+            if (edge) {
                 slot = P.ws_off + (long long)(L0 + e / NT) * NT + e % NT;
             } else {
-                const int d = src - edge_before - e_seen;  // rank among the rest
+                const int d = src - e;  // rank among the rest
                 const int q = d & (NT - 1);
                 slot = P.ws_off + (long long)(d >> 8) * NT + ((q & 3) << 6) +
                        (q >> 2);
