@@ -804,15 +804,19 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     const int q0 = min(wv * qlen, n), q1 = min(q0 + qlen, n);
     const unsigned long long below = (1ull << ln) - 1ull;
     int my_edges = 0;  // edge points of this wave's quarter (every lane)
+    int close = 0;     // points less than a bin after the point 4 before
     for (int i0 = q0; i0 < q1; i0 += 4 * 64) {
         double tr[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             tr[u] = b.t[off + min(i0 + u * 64 + ln, n - 1)];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            my_edges += __popcll(
-                __ballot(i0 + u * 64 + ln < q1 && is_edge(tr[u] - tmin)));
+        for (int u = 0; u < 4; ++u) {
+            const bool in = i0 + u * 64 + ln < q1;
+            my_edges += __popcll(__ballot(in && is_edge(tr[u] - tmin)));
+            const double t4 = __shfl_up(tr[u], 4);
+            close += (in && ln >= 4 && tr[u] - t4 < P.bd) ? 1 : 0;
+        }
     }
     __syncthreads();
     if (ln == 0) iscr[wv] = my_edges;
@@ -823,6 +827,12 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
         n_edge += iscr[k];
     }
     __syncthreads();
+    // Interleaved lanes (below) put points 4 apart side by side; where that
+    // is often less than a bin (dense irregular sampling), lanes of one
+    // atomic would hit the same bin, so such curves keep contiguous runs
+    // per lane (lane j owns points [j*L0, (j+1)*L0) of the rest), whose
+    // lanes are far apart in time.
+    const bool ilv = 50 * block_reduce(close, AddOp<int>(), iscr) < n;
     const int n0 = n - n_edge;
     const int L0 = (n0 + NT - 1) / NT, L1 = (n_edge + NT - 1) / NT;
     P.L = L0 + L1;
@@ -831,7 +841,8 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     const long long yq_pad = 0;
     // padding slots of this lane: zero weight, an in-range time
     for (int i = 0; i < L0; ++i)
-        if (i * NT + (tid & 63) * 4 + (tid >> 6) >= n0) {
+        if ((ilv ? i * NT + (tid & 63) * 4 + (tid >> 6) : tid * L0 + i) >=
+            n0) {
             const long long slot = P.ws_off + (long long)i * NT + tid;
             w.ws_t[slot] = t_pad;
             w.ws_w[slot] = 0;
@@ -871,8 +882,10 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
             } else {
                 const int d = src - e;  // rank among the rest
                 const int q = d & (NT - 1);
-                slot = P.ws_off + (long long)(d >> 8) * NT + ((q & 3) << 6) +
-                       (q >> 2);
+                slot = P.ws_off +
+                       (ilv ? (long long)(d >> 8) * NT + ((q & 3) << 6) +
+                                  (q >> 2)
+                            : (long long)(d % L0) * NT + d / L0);
             }
             w.ws_t[slot] = tp;
             // packed curves keep the packed bin increment in ws_w
