@@ -100,19 +100,35 @@ class DeviceBatch:
             return np.ascontiguousarray(np.concatenate(
                 [np.asarray(a, dtype=dt).ravel() for a in arrs]))
 
-        def up(a):
-            return torch.from_numpy(a).pin_memory().to(dev, non_blocking=True)
+        # one pinned staging buffer and one copy to the device; every array
+        # starts at a 256-byte offset and is a typed view of the device copy
+        arrays = [cat(ts, np.float64), cat(ys, y_dtype),
+                  None if dys is None else cat(dys, y_dtype), lc_off,
+                  cat(pers, np.float64), p_off, cat(durs, np.float64), d_off]
+        offs, total = [], 0
+        for a in arrays:
+            offs.append(total)
+            if a is not None:
+                total += (a.nbytes + 255) // 256 * 256
+        host = torch.empty(max(total, 256), dtype=torch.uint8,
+                           pin_memory=True)
+        hv = host.numpy()
+        for a, o in zip(arrays, offs):
+            if a is not None and a.nbytes:
+                hv[o:o + a.nbytes] = a.view(np.uint8)
+        buf = host.to(dev, non_blocking=True)
+
+        def piece(i):
+            a = arrays[i]
+            if a is None:
+                return None
+            tdt = torch.from_numpy(a[:0]).dtype
+            return buf[offs[i]:offs[i] + a.nbytes].view(tdt)
 
         b = DeviceBatch(
-            t=up(cat(ts, np.float64)),
-            y=up(cat(ys, y_dtype)),
-            dy=None if dys is None else up(cat(dys, y_dtype)),
-            lc_offsets=up(lc_off),
-            periods=up(cat(pers, np.float64)),
-            period_offsets=up(p_off),
-            durations=up(cat(durs, np.float64)),
-            dur_offsets=up(d_off),
-            host_period_offsets=p_off,
+            t=piece(0), y=piece(1), dy=piece(2), lc_offsets=piece(3),
+            periods=piece(4), period_offsets=piece(5), durations=piece(6),
+            dur_offsets=piece(7), host_period_offsets=p_off,
         )
         torch.cuda.current_stream(dev).synchronize()
         return b
@@ -145,7 +161,9 @@ class DeviceResult:
                             period_offsets=period_offsets)
 
     def to_host(self):
-        out = {k: getattr(self, k).cpu().numpy() for k in FIELDS}
+        # the seven FP64 fields in one copy
+        h = torch.stack([getattr(self, k) for k in FIELDS]).cpu().numpy()
+        out = {k: h[i] for i, k in enumerate(FIELDS)}
         out["best_index"] = self.best_index.cpu().numpy()
         out["status"] = self.status.cpu().numpy()
         return out
