@@ -61,6 +61,7 @@ constexpr int SUBMAX = 8;    // periods folded together in one pass
 constexpr int DMAX = 64;     // durations per light curve
 constexpr int EXACT_BINS = 1024;
 constexpr int NQ = 8;        // work queues = XCDs of an MI355X
+constexpr int EHIST = 1075 + 1024;  // k_prep's exponent histogram
 
 struct LcParams {
     double t_ref;     // min(t)
@@ -133,8 +134,8 @@ struct Work {
     unsigned int *big_next;    // [NQ] k_big's heads
     long long *big_p;          // [n_periods] those periods ...
     int *big_lc;               // ... and their curves
-    int *cand_lc;         // [n_lc * CAP] curve of each queued period
-    long long *cand_p;    // [n_lc * CAP] its global period index
+    int *cand_lc;         // [n_periods] curve of each queued period
+    long long *cand_p;    // [n_periods] its global period index
 };
 
 // ---------------------------------------------------------------- helpers
@@ -588,6 +589,10 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     __shared__ double dscr[NW];
     __shared__ int iscr[NW];
     __shared__ long long lscr[NW];
+    // exponent histogram of |(y - median) * ivar|: [0] zeros, [e + 1075]
+    // values with ilogb e (every finite double: -1074 <= e <= 1023)
+    __shared__ unsigned int ehist[EHIST];
+    __shared__ int s_e50;
 
     const int lc = blockIdx.x;
     const long long off = b.lc_off[lc];
@@ -723,7 +728,10 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
         P.seq_iv = si;
     }
 
-    // pass B: magnitudes for the fixed-point scales, weight uniformity
+    // pass B: magnitudes for the fixed-point scales, weight uniformity, and
+    // the exponent histogram of the flux magnitudes
+    for (int i = tid; i < EHIST; i += NT) ehist[i] = 0u;
+    __syncthreads();
     double sw_sum = 0.0, sy_sum = 0.0, iv_min = INFINITY, iv_max = -INFINITY;
     for (int i0 = tid; i0 < n; i0 += 4 * NT) {
         double yr[4], dr[4];
@@ -736,16 +744,38 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
             double iv = 1.0;
             if (b.dy) iv = 1.0 / (dr[u] * dr[u]);
             double yc = yr[u] - med;
+            const double a = fabs(yc * iv);
             sw_sum += iv;
-            sy_sum += fabs(yc * iv);
+            sy_sum += a;
             iv_min = fmin(iv_min, iv);
             iv_max = fmax(iv_max, iv);
+            atomicAdd(&ehist[a > 0.0 && a <= DBL_MAX ? ilogb(a) + 1075 : 0],
+                      1u);
         }
     }
     sw_sum = block_reduce(sw_sum, AddOp<double>(), dscr);
     sy_sum = block_reduce(sy_sum, AddOp<double>(), dscr);
     iv_min = block_reduce(iv_min, MinOp(), dscr);
     iv_max = block_reduce(iv_max, MaxOp(), dscr);
+    // e50: exponent bin of the median magnitude (the first bin whose
+    // cumulative count reaches half the points)
+    {
+        constexpr int PER = (EHIST + NT - 1) / NT;
+        const int h0 = min(tid * PER, EHIST), h1 = min(h0 + PER, EHIST);
+        unsigned int c = 0;
+        for (int i = h0; i < h1; ++i) c += ehist[i];
+        unsigned int tot;
+        unsigned int before = block_excl_scan(c, uscr, &tot);
+        const unsigned int half = (unsigned int)(n + 1) / 2;
+        if (tid == 0) s_e50 = 0;
+        __syncthreads();
+        for (int i = h0; i < h1; ++i) {
+            const unsigned int after = before + ehist[i];
+            if (before < half && after >= half) s_e50 = i;
+            before = after;
+        }
+        __syncthreads();
+    }
     // Uniform weights: a bin is (point count, weight*flux sum) packed into
     // one int64 as count << ybits + sum.  Prefix sums (with the padding
     // copies) reach at most 2n points, which takes cbits bits; the flux
@@ -756,7 +786,17 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     // refinement widens its near-tie window by that bound (k_cands).  In
     // parity mode packing is limited to n <= 65536 points, so that the
     // error stays near 1e-9 and the window small.
-    P.packed = (iv_min == iv_max && 63 - cbits >= 40 &&
+    // The packed flux quantum is proportional to the sum of magnitudes, so
+    // one extreme value (an outlier far above the noise) coarsens every
+    // window sum.  Packing is kept only while that sum stays within 64 times
+    // n points of the median magnitude (R = sum / (n * 2^e50) is ~1.2-2.4
+    // for Gaussian noise, ~100+ with a single 1e6-sigma point at 15k
+    // points); beyond it the 60-bit unpacked bins keep the error at the
+    // noise-only level.
+    const bool spread_ok =
+        s_e50 > 0 ? sy_sum <= ldexp(64.0 * (double)n, s_e50 - 1075)
+                  : !(sy_sum > 0.0);
+    P.packed = (iv_min == iv_max && 63 - cbits >= 40 && spread_ok &&
                 (b.mode == ATG_MODE_FAST || n <= 65536)) ? 1 : 0;
     P.ybits = 63 - cbits;
     // scale so that the sum of |quantized| values stays below 2^60 (packed:
@@ -1934,9 +1974,10 @@ namespace {
 
 // Periods whose best windows hold the same points have equal int64 power,
 // so the near-tie set can be a long plateau of neighbouring periods (59 at
-// the 5,000-period benchmark grid); all of it must be recomputed, or an
-// unrefined member (exact integer value) outranks the refined ones.
-constexpr int CAP = 1024;          // refined periods per curve
+// the 5,000-period benchmark grid, every period of a sparse curve whose best
+// window holds one deep point); all of it is recomputed, or an unrefined
+// member (exact integer value) could outrank the refined ones.  The queue
+// holds up to every period of the batch, so nothing is dropped.
 constexpr double TIE_REL = 1e-9;   // >> relative error of the int64 path
 // k_exact's FP64 histogram: one wave per queued period walks its points
 // sequentially, so the kernel is latency-bound; a small LDS histogram (16 KB)
@@ -1946,7 +1987,6 @@ constexpr int EXACT_WG_PER_CU = 8;
 __global__ __launch_bounds__(NT) void k_cands(Batch b, Out o, Work w) {
     __shared__ double dscr[NW];
     __shared__ int iscr[NW];
-    __shared__ long long s_c[CAP];
     __shared__ unsigned int s_base;
     const int lc = blockIdx.x;
     const long long po = b.per_off[lc], np = b.per_off[lc + 1] - po;
@@ -1984,26 +2024,29 @@ __global__ __launch_bounds__(NT) void k_cands(Batch b, Out o, Work w) {
     m = block_reduce(m, MaxOp(), dscr);
     if (!(m > -INFINITY)) return;
     const double thr = m - TIE_REL * fabs(m);
+    auto cand = [&](long long i) -> bool {
+        if (i >= np) return false;
+        const double v = o.power[po + i];
+        return v > -INFINITY && v + bound(po + i) * fabs(v) >= thr;
+    };
+    // count, reserve the curve's slots, then write them in period order
     int count = 0;
-    for (long long base = 0; base < np && count < CAP; base += NT) {
-        const long long i = base + threadIdx.x;
-        int f = 0;
-        if (i < np) {
-            const double v = o.power[po + i];
-            f = (v > -INFINITY && v + bound(po + i) * fabs(v) >= thr) ? 1 : 0;
-        }
-        int tot;
-        const int e = block_excl_scan(f, iscr, &tot);
-        if (f && count + e < CAP) s_c[count + e] = po + i;
-        count += tot;
-    }
-    count = min(count, CAP);
-    __syncthreads();
+    for (long long i = threadIdx.x; i < np; i += NT) count += cand(i) ? 1 : 0;
+    count = block_reduce(count, AddOp<int>(), iscr);
     if (threadIdx.x == 0) s_base = atomicAdd(w.cand_count, (unsigned)count);
     __syncthreads();
-    for (int k = threadIdx.x; k < count; k += NT) {
-        w.cand_lc[s_base + k] = lc;
-        w.cand_p[s_base + k] = s_c[k];
+    const unsigned int base0 = s_base;
+    int done = 0;
+    for (long long base = 0; base < np; base += NT) {
+        const long long i = base + threadIdx.x;
+        const int f = cand(i) ? 1 : 0;
+        int tot;
+        const int e = block_excl_scan(f, iscr, &tot);
+        if (f) {
+            w.cand_lc[base0 + done + e] = lc;
+            w.cand_p[base0 + done + e] = po + i;
+        }
+        done += tot;
     }
 }
 
@@ -3015,8 +3058,9 @@ Layout layout_of(const atg_bls_batch *b) {
     L.queues = o;  o = align256(o + NQ * (sizeof(long long) + sizeof(int)));
     L.counter = o; o = align256(o + 3 * sizeof(unsigned int));
     L.big_q = o;   o = align256(o + NQ * (sizeof(long long) + 2 * sizeof(int)));
-    L.cand_lc = o; o = align256(o + nlc * CAP * sizeof(int));
-    L.cand_p = o;  o = align256(o + nlc * CAP * sizeof(long long));
+    const size_t nper = (size_t)b->n_periods;
+    L.cand_lc = o; o = align256(o + nper * sizeof(int));
+    L.cand_p = o;  o = align256(o + nper * sizeof(long long));
     L.t = o;       o = align256(o + slots * sizeof(double));
     L.w = o;       o = align256(o + slots * sizeof(long long));
     L.y = o;       o = align256(o + slots * sizeof(long long));
@@ -3024,7 +3068,6 @@ Layout layout_of(const atg_bls_batch *b) {
     L.nt = o;      o = align256(o + npt * sizeof(double));
     L.nw = o;      o = align256(o + npt * sizeof(long long));
     L.ny = o;      o = align256(o + npt * sizeof(long long));
-    const size_t nper = (size_t)b->n_periods;
     L.tk_pos = o;  o = align256(o + nper * sizeof(int));
     L.tk_st = o;   o = align256(o + nper);
     L.big_p = o;   o = align256(o + nper * sizeof(long long));

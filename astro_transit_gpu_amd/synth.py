@@ -81,6 +81,32 @@ def config2_curve(rng, n_points=15000, baseline=27.0, sigma=1e-3):
             truth)
 
 
+def outlier_curve(rng, n_points=15000, sigmas=1e6, hetero=False, sign=1.0,
+                  sigma=1e-3):
+    """A curve of the 1,000-curve batch (2-min grid over 27 d, two gaps, one
+    injected box) with one flux outlier ``sigmas`` standard deviations from
+    the mean at a random point, FP64 y/dy.  ``hetero``: dy varies by +-10%
+    from point to point (otherwise uniform dy).  A single extreme value sets
+    the magnitude of the flux sums, which is what a fixed-point binning has
+    to survive."""
+    gaps = []
+    for _ in range(2):
+        ln = rng.uniform(0.5, 1.5)
+        a = rng.uniform(0.0, 27.0 - ln)
+        gaps.append((a, a + ln))
+    t = sampled_grid(rng, n_points, 27.0, gaps)
+    y = 1.0 + sigma * rng.standard_normal(t.size)
+    period = float(rng.uniform(0.7, 9.0))
+    y = inject_box(t, y, period, float(rng.uniform(0.0, period)),
+                   float(rng.uniform(0.05, 0.2)),
+                   float(np.exp(rng.uniform(np.log(1e-3), np.log(5e-3)))))
+    y[int(rng.integers(0, t.size))] += sign * sigmas * sigma
+    dy = np.full(t.size, sigma)
+    if hetero:
+        dy = dy * rng.uniform(0.9, 1.1, t.size)
+    return t, y, dy
+
+
 def config2(n_lc=1000, seed=2, n_points=15000, n_periods=5000):
     """Batch of ``n_lc`` curves x ``n_points``; 5,000 log-uniform periods
     0.5-13.5 d shared by all curves; five durations 0.04-0.20 d."""

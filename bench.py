@@ -284,8 +284,10 @@ def precision_leg(n_lc, n_sample, objective, seconds_cap=None):
             "log-uniform periods 1-100 d x 6 durations 0.04-0.3 d, %s" % (
                 n_lc, objective),
             "compared_curves": n_sample, "gpu": gpu,
-            "errors": {k: _summarise(v) for k, v in stats.items()},
-            "cpu_oracle_LC_per_s": round(n_sample / cpu_s, 4),
+            "errors": {k: _summarise(v) for k, v in stats.items()}
+            if n_sample else None,
+            "cpu_oracle_LC_per_s": round(n_sample / cpu_s, 4)
+            if n_sample else None,
             "cpu_cores": cores, "generation_s": round(gen, 1)}
 
 

@@ -3,7 +3,9 @@
 Run ONCE in the build container with the conda interpreter that carries
 astropy 4.3.1:
 
-    /opt/conda/bin/python3.9 tests/golden/make_golden.py
+    /opt/conda/bin/python3.9 tests/golden/make_golden.py [NAME ...]
+
+(with names, only those fixtures are rewritten).
 
 What it runs.  The oracle of the hot path is astropy's
 ``BoxLeastSquares.power(method="fast")`` (the reference's CPU path,
@@ -81,8 +83,14 @@ def astropy_power(t, y, dy, period, duration, objective="likelihood",
     return dict(zip(FIELDS, res))
 
 
+ONLY = set(sys.argv[1:])  # names to (re)write; empty: every fixture
+
+
 def save(name, t, y, dy, periods, durations, objective="likelihood",
          oversample=10, **extra):
+    if ONLY and name not in ONLY:
+        return astropy_power(t, y, dy, periods, durations, objective,
+                             oversample)
     out = astropy_power(t, y, dy, periods, durations, objective, oversample)
     arrs = dict(t=np.asarray(t), y=np.asarray(y),
                 periods=np.atleast_1d(np.asarray(periods, dtype=np.float64)),
@@ -233,6 +241,18 @@ def main():
         per = SYNTH.capped_grid(t, durs, 50, 400)
         obj = "likelihood" if i % 2 == 0 else "snr"
         save("ragged_%d" % i, t, y, dy, per, durs, obj)
+
+    # 7. One extreme flux outlier (1e4 and 1e6 sigma, above and below the
+    #    curve), with uniform dy and with dy varied +-10% per point.
+    r = np.random.default_rng(15)
+    for name, sig, het, sgn in (("outlier_1e4", 1e4, False, 1.0),
+                                ("outlier_1e6", 1e6, False, 1.0),
+                                ("outlier_1e6_low", 1e6, False, -1.0),
+                                ("outlier_1e4_hetero", 1e4, True, 1.0),
+                                ("outlier_1e6_hetero", 1e6, True, 1.0)):
+        t, y, dy = SYNTH.outlier_curve(r, 4000, sig, het, sgn)
+        save(name, t, y, dy, np.geomspace(0.5, 13.5, 300),
+             SYNTH.DURATIONS_5)
 
 
 if __name__ == "__main__":

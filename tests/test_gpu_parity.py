@@ -300,8 +300,8 @@ def test_long_plateau_of_tied_periods(ctx):
 
 def test_refinement_queue_beyond_one_workgroup(ctx):
     """Flat flux: every window has depth 0, so all 1,500 periods tie at
-    power 0 and the refinement queues its full per-curve capacity (1,024,
-    more than one workgroup's threads).  The oracle's answer (first period,
+    power 0 and the refinement queues every one of them (more than one
+    workgroup's threads).  The oracle's answer (first period,
     all fields) must come back, and a second curve in the same batch must
     be unaffected."""
     from astro_transit_gpu_amd import DeviceBatch, synth
@@ -326,3 +326,83 @@ def test_refinement_queue_beyond_one_workgroup(ctx):
         assert int(h["best_index"][i]) == ib
         for k in ref:
             assert g[k][ib] == ref[k][ib], (i, k)
+
+
+def plateau_beyond_1024():
+    """A sparse curve (20 points over 27 d) whose best window holds only one
+    deep point for nearly every period: the exact (integer) power is the
+    same on ~17,800 of 20,000 periods, and the oracle's FP64 rounding gives
+    three slightly different values.  The grid drops every period of the
+    largest value among the first 1,500 plateau members, so the oracle's
+    maximum lies thousands of plateau members in."""
+    rng = np.random.default_rng(2024)
+    n = 20
+    t = np.sort(rng.uniform(0.0, 27.0, n))
+    y = 1.0 + 1e-3 * rng.standard_normal(n)
+    y[n // 2] = 0.9
+    dy = np.full(n, 1e-3)
+    dense = np.geomspace(0.5, 13.5, 20000)
+    durs = np.array([0.04, 0.08, 0.12, 0.16, 0.20])
+    p = O.power(t, y, dy, dense, durs, "likelihood")["power"]
+    near = np.nonzero(p >= p.max() * (1 - 1e-9))[0]
+    keep = np.ones(p.size, bool)
+    keep[:near[1500]] &= p[:near[1500]] != p.max()
+    return t, y, dy, dense[keep], durs
+
+
+def test_refinement_beyond_1024_tied_periods(ctx):
+    """Every near-tied period is recomputed in the oracle's order, however
+    many there are: here the oracle's maximum is the ~3,900th member of a
+    ~17,800-period plateau, so a refinement limited to the first 1,024
+    members would return another period."""
+    from astro_transit_gpu_amd import DeviceBatch
+    t, y, dy, periods, durs = plateau_beyond_1024()
+    ref = O.power(t, y, dy, periods, durs, "likelihood")
+    mx = ref["power"].max()
+    near = np.nonzero(ref["power"] >= mx * (1 - 1e-9))[0]
+    ib = int(np.argmax(ref["power"]))
+    rank = int(np.searchsorted(near, ib))
+    print("plateau %d periods, oracle best %d (plateau member %d)" % (
+        near.size, ib, rank), flush=True)
+    assert rank > 1024
+    b = DeviceBatch.from_host([t], [y], [dy], periods, durs,
+                              device=ctx.device)
+    h = ctx.run(b, objective="likelihood").to_host()
+    assert int(h["status"][0]) == 0
+    assert_parity(h, ref, "plateau beyond 1024")
+    assert int(h["best_index"][0]) == ib
+    for k in ref:
+        assert h[k][ib] == ref[k][ib], k
+
+
+OUTLIER_CASES = [(1e4, False, 1.0), (1e6, False, 1.0), (1e6, False, -1.0),
+                 (1e8, False, 1.0), (1e4, True, 1.0), (1e6, True, 1.0)]
+
+
+@pytest.mark.parametrize("sigmas,hetero,sign", OUTLIER_CASES,
+                         ids=lambda v: str(v))
+def test_flux_outlier_parity(ctx, sigmas, hetero, sign):
+    """A 15,000-point curve with one flux outlier of 1e4 to 1e8 sigma (above
+    or below the curve), uniform dy (packed bins unless the outlier would
+    coarsen their flux quantum) or dy varied per point (unpacked bins):
+    every period within the parity bar of the FP64 oracle, the best period
+    and all its fields the oracle's."""
+    from astro_transit_gpu_amd import DeviceBatch, synth
+    rng = np.random.default_rng(int(np.log10(sigmas)) * 10 + hetero +
+                                (2 if sign < 0 else 0))
+    t, y, dy = synth.outlier_curve(rng, 15000, sigmas, hetero, sign)
+    periods = np.geomspace(0.5, 13.5, 1000)
+    durs = synth.DURATIONS_5
+    print("outlier %g sigma, hetero %s, sign %+d" % (sigmas, hetero, sign),
+          flush=True)
+    ref = O.power(t, y, dy, periods, durs, "likelihood")
+    b = DeviceBatch.from_host([t], [y], [dy], periods, durs,
+                              device=ctx.device)
+    h = ctx.run(b, objective="likelihood").to_host()
+    assert int(h["status"][0]) == 0
+    rel = assert_parity(h, ref, "outlier")
+    print("max rel power err %.3e" % rel, flush=True)
+    ib = int(np.argmax(ref["power"]))
+    assert int(h["best_index"][0]) == ib
+    for k in ref:
+        assert h[k][ib] == ref[k][ib], k
