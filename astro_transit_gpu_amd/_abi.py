@@ -24,6 +24,8 @@ LC_STATUS = {
 
 ATG_F32 = 1
 ATG_F64 = 2
+# static device workspace of atg_bls_run_noalloc
+ATG_NOALLOC_SCRATCH_BYTES = 64 << 20
 OBJECTIVES = {"likelihood": 0, "snr": 1}
 MODES = {"parity": 0, "fast": 1}
 
@@ -38,6 +40,7 @@ EXPORTS = (
     "atg_ctx_get_tuning", "atg_bls_workspace_size", "atg_bls_run",
     "atg_bls_run_sync", "atg_timing_main_kernel", "atg_bls_top_k",
     "atg_bls_stats", "atg_bls_model", "atg_bls_run_best",
+    "atg_bls_run_noalloc",
     "atg_bls_workspace_size_best",
 )
 
@@ -112,7 +115,7 @@ def lib():
     L.atg_bls_workspace_size.restype = ctypes.c_int
     L.atg_bls_workspace_size.argtypes = [_vp, ctypes.POINTER(Batch),
                                          ctypes.POINTER(ctypes.c_uint64)]
-    for fn in (L.atg_bls_run, L.atg_bls_run_sync):
+    for fn in (L.atg_bls_run, L.atg_bls_run_sync, L.atg_bls_run_noalloc):
         fn.restype = ctypes.c_int
         fn.argtypes = [_vp, ctypes.POINTER(Batch), ctypes.POINTER(Out)]
     L.atg_bls_run_best.restype = ctypes.c_int

@@ -48,6 +48,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -3010,6 +3011,48 @@ __global__ __launch_bounds__(NT) void k_final(Batch b, Out o, Work w,
 
 // ===================================================================== ABI
 
+// The workspace of atg_bls_run_noalloc: a static device array, so that the
+// call allocates nothing.  Every GPU the library runs on holds its own copy
+// at its own address until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it names a stand-in, not device
+// memory): each device's address comes from hipGetSymbolAddress, resolved
+// once per device at context creation.
+static __device__ __align__(256) unsigned char scratch[ATG_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on `device` (resolved with that device
+// current, the caller's current device restored); nullptr on failure.
+void *scratch_on(int device, hipError_t *err) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    *err = hipSuccess;
+    if (device < 0 || device >= MAX_DEVICES) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    if (g_scratch_ptr[device]) return g_scratch_ptr[device];
+    int prev = 0;
+    hipError_t e = hipGetDevice(&prev);
+    if (e == hipSuccess) e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        *err = e;
+        return nullptr;
+    }
+    void *ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    hipSetDevice(prev);
+    if (e != hipSuccess || !ptr) {
+        *err = e != hipSuccess ? e : hipErrorInvalidSymbol;
+        return nullptr;
+    }
+    g_scratch_ptr[device] = ptr;
+    return ptr;
+}
+}  // namespace
+
 struct atg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -3156,14 +3199,22 @@ Batch to_batch(const atg_bls_batch *b) {
 // The launch sequence of atg_bls_run (bst == nullptr: o holds the caller's
 // per-period arrays) and atg_bls_run_best (o == nullptr: the per-period
 // arrays live in the workspace, k_final gathers the winners into bst).
+// ws == nullptr: the context's workspace (grown on demand); otherwise a
+// workspace of at least the layout's size that the caller provides (the
+// static scratch of atg_bls_run_noalloc: then nothing is allocated, and no
+// timing events are recorded).
 int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
-             const atg_bls_best *bst) {
+             const atg_bls_best *bst, char *ws = nullptr) {
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_fail(c, e, "hipSetDevice");
     const Layout L = layout_of(b);
-    int rc = ensure_workspace(c, L.total + (bst ? best_bytes(b) : 0));
-    if (rc != ATG_OK) return rc;
-    char *base = (char *)c->ws;
+    const bool own_ws = ws == nullptr;
+    if (own_ws) {
+        int rc = ensure_workspace(c, L.total + (bst ? best_bytes(b) : 0));
+        if (rc != ATG_OK) return rc;
+    }
+    char *base = own_ws ? (char *)c->ws : ws;
+    const bool timing = c->timing && own_ws;
     Work w;
     w.prm = (LcParams *)(base + L.prm);
     w.lc_flags = (int *)(base + L.flags);
@@ -3248,7 +3299,7 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
     hipLaunchKernelGGL(k_items, dim3(1), dim3(NT), 0, c->stream,
                        (long long)b->n_lc, kb.per_off, per_item, w);
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (c->timing) {
+    if (timing) {
         for (int i = 0; i < 2; ++i) {
             hipEvent_t ev;
             if (!c->spare.empty()) {
@@ -3274,7 +3325,7 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
     hipLaunchKernelGGL(k_big, dim3(grid), dim3(NT), (size_t)big_bins * 16,
                        c->stream, kb, ko, w, big_bins);
     hipLaunchKernelGGL(k_fields, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w);
-    if (c->timing) {  // the periodogram stage: k_main, k_big and k_fields
+    if (timing) {  // the periodogram stage: k_main, k_big and k_fields
         hipEventRecord(ev1, c->stream);
         c->pending.emplace_back(ev0, ev1);
     }
@@ -3308,6 +3359,11 @@ int atg_ctx_create(int device, void *stream, uint64_t hbm_budget_bytes,
     if (hipGetDeviceProperties(&prop, device) != hipSuccess)
         return ATG_ERR_DEVICE;
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return ATG_ERR_DEVICE;
+    hipError_t se;
+    if (!scratch_on(device, &se)) {
+        (void)hipGetLastError();
+        return ATG_ERR_HIP;
+    }
     atg_ctx *c = new atg_ctx();
     c->device = device;
     c->stream = (hipStream_t)stream;
@@ -3412,6 +3468,28 @@ int atg_bls_run_sync(atg_ctx *c, const atg_bls_batch *b,
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "hipStreamSynchronize");
     return ATG_OK;
+}
+
+int atg_bls_run_noalloc(atg_ctx *c, const atg_bls_batch *b,
+                        const atg_bls_out *o) {
+    if (!c) return ATG_ERR_ARG;
+    int rc = check_batch(c, b);
+    if (rc != ATG_OK) return rc;
+    if (!o || !o->power || !o->depth || !o->depth_err || !o->duration ||
+        !o->transit_time || !o->depth_snr || !o->log_likelihood)
+        return fail(c, ATG_ERR_ARG, "null output array");
+    const size_t need = layout_of(b).total;
+    if (need > (size_t)ATG_NOALLOC_SCRATCH_BYTES)
+        return fail(c, ATG_ERR_BUDGET,
+                    "atg_bls_run_noalloc: the batch needs a workspace of " +
+                        std::to_string(need) +
+                        " bytes, more than the static scratch of " +
+                        std::to_string((size_t)ATG_NOALLOC_SCRATCH_BYTES) +
+                        " bytes; use atg_bls_run");
+    hipError_t e;
+    void *ws = scratch_on(c->device, &e);
+    if (!ws) return hip_fail(c, e, "hipGetSymbolAddress(scratch)");
+    return run_impl(c, b, o, nullptr, (char *)ws);
 }
 
 int atg_bls_top_k(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,

@@ -323,6 +323,26 @@ class Context:
         self._check(fn(self._h, ctypes.byref(b), ctypes.byref(o)))
         return out
 
+    def run_noalloc(self, batch: DeviceBatch, objective="likelihood",
+                    mode="parity", oversample=10,
+                    out: Optional[DeviceResult] = None, sync=True):
+        """``run`` through atg_bls_run_noalloc: the library allocates no
+        device memory (its workspace is a static array of
+        ``_abi.ATG_NOALLOC_SCRATCH_BYTES`` per GPU, shared by every call on
+        that GPU, so overlapping calls race).  Results equal ``run``'s bit
+        for bit; a batch whose workspace exceeds the scratch raises
+        MemoryBudgetError.  Pass ``out`` to allocate nothing on this side
+        either."""
+        if out is None:
+            out = DeviceResult.empty(batch.n_periods, batch.n_lc,
+                                     batch.host_period_offsets, self.device)
+        b, o = self._structs(batch, out, objective, mode, oversample)
+        self._check(self._lib.atg_bls_run_noalloc(self._h, ctypes.byref(b),
+                                                  ctypes.byref(o)))
+        if sync:
+            self.stream.synchronize()
+        return out
+
     def run_best(self, batch: DeviceBatch, objective="likelihood",
                  mode="parity", oversample=10, out: Optional[DeviceBest] = None,
                  sync=True):

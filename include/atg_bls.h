@@ -167,6 +167,25 @@ int atg_bls_run(atg_ctx *ctx, const atg_bls_batch *batch,
 int atg_bls_run_sync(atg_ctx *ctx, const atg_bls_batch *batch,
                      const atg_bls_out *out);
 
+/* Size of the static device workspace of atg_bls_run_noalloc (per GPU,
+ * held from the library's first context on that GPU until the process
+ * ends). */
+#define ATG_NOALLOC_SCRATCH_BYTES (64ull << 20)
+
+/* atg_bls_run without any device allocation: the same launch sequence and
+ * results bit for bit, with the library's workspace in a static device
+ * array of ATG_NOALLOC_SCRATCH_BYTES instead of memory the context
+ * allocates.  No counterpart in the reference; it is for callers that
+ * cannot allocate: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  The caller still passes the batch and
+ * the outputs.  A batch whose workspace (atg_bls_workspace_size) exceeds
+ * the scratch returns ATG_ERR_BUDGET before anything is enqueued.  Every
+ * call on one GPU shares that GPU's scratch (any context): calls that
+ * overlap on one GPU race, so serialise them (one stream, or events).
+ * No timing events are recorded for these calls. */
+int atg_bls_run_noalloc(atg_ctx *ctx, const atg_bls_batch *batch,
+                        const atg_bls_out *out);
+
 /* Caller-allocated device outputs of a best-only run: one entry per light
  * curve, the fields of its best period (the period at the index
  * atg_bls_out.best_index would hold; NaN fields for a curve without
