@@ -40,7 +40,7 @@ EXPORTS = (
     "atg_ctx_get_tuning", "atg_bls_workspace_size", "atg_bls_run",
     "atg_bls_run_sync", "atg_timing_main_kernel", "atg_bls_top_k",
     "atg_bls_stats", "atg_bls_model", "atg_bls_run_best",
-    "atg_bls_run_noalloc",
+    "atg_bls_run_noalloc", "atg_ctx_pin_workspace",
     "atg_bls_workspace_size_best",
 )
 
@@ -104,6 +104,8 @@ def lib():
                                  ctypes.POINTER(_vp)]
     L.atg_ctx_destroy.restype = None
     L.atg_ctx_destroy.argtypes = [_vp]
+    L.atg_ctx_pin_workspace.restype = ctypes.c_int
+    L.atg_ctx_pin_workspace.argtypes = [_vp, ctypes.c_int]
     L.atg_ctx_set_stream.restype = ctypes.c_int
     L.atg_ctx_set_stream.argtypes = [_vp, _vp]
     L.atg_last_error.restype = ctypes.c_char_p

@@ -3060,6 +3060,7 @@ struct atg_ctx {
     std::string err;
     void *ws = nullptr;
     size_t ws_bytes = 0;
+    int pins = 0;  // captured graphs that hold ws: it must not move
     atg_tuning tune{0, 2540, 4, 0};  // periods per item: per batch
     int num_cu = 0;
     size_t lds_max = 65536;
@@ -3152,6 +3153,15 @@ size_t best_bytes(const atg_bls_batch *b) {
 
 int ensure_workspace(atg_ctx *c, size_t need) {
     if (need <= c->ws_bytes) return ATG_OK;
+    if (c->pins > 0)  // a captured graph holds the current address
+        return fail(c, ATG_ERR_BUDGET,
+                    "workspace of " + std::to_string(need) +
+                        " bytes needed, but the context's " +
+                        std::to_string(c->ws_bytes) +
+                        "-byte workspace is pinned by " +
+                        std::to_string(c->pins) +
+                        " captured graph(s); release them or use another "
+                        "context");
     size_t want = need + need / 4;
     if (c->budget && want > c->budget) want = need;
     if (c->budget && need > c->budget)
@@ -3386,6 +3396,13 @@ void atg_ctx_destroy(atg_ctx *c) {
     for (auto e : c->spare) hipEventDestroy(e);
     if (c->ws) hipFree(c->ws);
     delete c;
+}
+
+int atg_ctx_pin_workspace(atg_ctx *c, int pin) {
+    if (!c) return ATG_ERR_ARG;
+    if (pin) ++c->pins;
+    else if (c->pins > 0) --c->pins;
+    return c->pins;
 }
 
 int atg_ctx_set_stream(atg_ctx *c, void *stream) {

@@ -364,7 +364,12 @@ class Context:
                 objective="likelihood", mode="parity", oversample=10):
         """Capture one run of ``batch`` into ``out`` as a HIP graph
         (torch.cuda.CUDAGraph).  The workspace is sized by an eager run first;
-        the captured launch sequence holds no memset or allocation."""
+        the captured launch sequence holds no memset or allocation.  The
+        graph holds the workspace's address, so the workspace is pinned
+        until :meth:`CapturedRun.release`: a later call of this context that
+        would need more raises MemoryBudgetError instead of moving it.
+        Replays and eager calls of this context share the workspace and must
+        not overlap (use another Context for concurrent eager work)."""
         self.run(batch, objective, mode, oversample, out=out, sync=True)
         g = torch.cuda.CUDAGraph()
         prev = self.stream
@@ -376,7 +381,8 @@ class Context:
                          sync=False)
         finally:
             self.set_stream(prev)
-        return g
+        self._lib.atg_ctx_pin_workspace(self._h, 1)
+        return CapturedRun(self, g)
 
     def top_k(self, batch: DeviceBatch, result: DeviceResult, k=5,
               distance=10, sync=True):
@@ -510,6 +516,33 @@ class Context:
         self._check(self._lib.atg_bls_workspace_size_best(
             self._h, ctypes.byref(b), ctypes.byref(n)))
         return int(n.value)
+
+
+class CapturedRun:
+    """A captured launch sequence (``Context.capture``) and the pin it holds
+    on its context's workspace."""
+
+    def __init__(self, ctx: Context, graph):
+        self.ctx = ctx
+        self.graph = graph
+
+    def replay(self):
+        if self.graph is None:
+            raise RuntimeError("captured run was released")
+        self.graph.replay()
+
+    def release(self):
+        """Drop the graph and unpin the workspace (idempotent)."""
+        if self.graph is not None:
+            self.graph = None
+            if getattr(self.ctx, "_h", None):
+                self.ctx._lib.atg_ctx_pin_workspace(self.ctx._h, 0)
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
 
 
 _default_ctx = {}

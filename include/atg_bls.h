@@ -140,6 +140,15 @@ void atg_ctx_destroy(atg_ctx *ctx);
 /* Replaces the stream the context enqueues on. */
 int atg_ctx_set_stream(atg_ctx *ctx, void *stream);
 
+/* Pins (pin != 0) or releases one pin of (pin == 0) the context's
+ * workspace; returns the number of pins held.  No counterpart in the
+ * reference.  A HIP graph captured from this context's launches holds the
+ * workspace's address: while pinned, a call that would need a larger
+ * workspace fails with ATG_ERR_BUDGET instead of moving it.  Replays and
+ * eager calls of one context share that workspace, so they must not
+ * overlap (one stream, or a separate context for eager work). */
+int atg_ctx_pin_workspace(atg_ctx *ctx, int pin);
+
 /* Last error message of the context (never NULL). */
 const char *atg_last_error(const atg_ctx *ctx);
 

@@ -207,10 +207,12 @@ def test_fast_mode_drift(ctx, name):
         rel, window_agreement(a, c["out"])))
 
 
-def test_hipgraph_replays_match_eager(ctx):
+def test_hipgraph_replays_match_eager():
     """The launch sequence captured as a HIP graph gives the eager result on
     every replay, with eager work and synchronisations in between."""
-    from astro_transit_gpu_amd import DeviceBatch, DeviceResult, synth
+    from astro_transit_gpu_amd import (Context, DeviceBatch, DeviceResult,
+                                       synth)
+    ctx = Context(0)  # its own workspace, sized by this test's batches
     d = synth.config2(n_lc=8, seed=23, n_points=5000, n_periods=600)
     b = DeviceBatch.from_host(d["t"], d["y"], d["dy"], d["periods"],
                               d["durations"], device=ctx.device)
@@ -229,6 +231,17 @@ def test_hipgraph_replays_match_eager(ctx):
             assert np.array_equal(h[k], eager[k]), (rep, k)
         ctx.run(b, objective="snr")  # unrelated eager work between replays
         torch.cuda.synchronize()
+    # the graph pins the workspace: a batch needing more is refused until
+    # the graph is released, then runs
+    from astro_transit_gpu_amd._abi import MemoryBudgetError
+    d2 = synth.config2(n_lc=16, seed=24, n_points=6000, n_periods=600)
+    b2 = DeviceBatch.from_host(d2["t"], d2["y"], d2["dy"], d2["periods"],
+                               d2["durations"], device=ctx.device)
+    assert ctx.workspace_bytes(b2) > ctx.workspace_bytes(b)
+    with pytest.raises(MemoryBudgetError, match="pinned"):
+        ctx.run(b2, objective="likelihood")
+    g.release()
+    assert int(ctx.run(b2, objective="likelihood").status.cpu().max()) == 0
 
 
 @pytest.mark.parametrize("mode", ["parity", "fast"])
