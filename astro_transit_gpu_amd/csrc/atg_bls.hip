@@ -63,6 +63,9 @@ constexpr int DMAX = 64;     // durations per light curve
 constexpr int EXACT_BINS = 1024;
 constexpr int NQ = 8;        // work queues = XCDs of an MI355X
 constexpr int EHIST = 1075 + 1024;  // k_prep's exponent histogram
+// entries past a period's prefix arrays that k_main's box scan reads (the
+// starts of a duration's last stride beyond its final window)
+constexpr int PADN = NT - 1;
 
 struct LcParams {
     double t_ref;     // min(t)
@@ -479,11 +482,6 @@ __device__ __forceinline__ void bin_points(
                 }
         }
     }
-}
-
-// Sign-extended weight*flux field and point count of a packed bin value.
-__device__ __forceinline__ long long packed_y(long long v, int ybits) {
-    return (long long)((unsigned long long)v << (64 - ybits)) >> (64 - ybits);
 }
 
 // ------------------------------------------------------------------ k_prep
@@ -946,9 +944,13 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     // holds at most EXACT_BINS bins
     P.natural = 0;
     const double nbmax = ceil(pmax / P.bd) + b.oversample;
-    const bool big = P.packed
-                         ? nbmax + 1 + floor((nbmax + 2) / 2) > 2.0 * lds_bins
-                         : nbmax + 1 > (double)lds_bins;
+    // as k_main's capacity test for the longest period alone (with the
+    // overhang of its box scan)
+    const double ov =
+        fmax(0.0, PADN + fmin(round(dmax / P.bd), 32767.0) - nbmax);
+    const bool big = P.packed ? nbmax + 1 + fmax(floor((nbmax + 1 + ov + 1) / 2),
+                                                 ov) > 2.0 * lds_bins
+                              : nbmax + 1 + ov > (double)lds_bins;
     const bool big_exact = b.mode == ATG_MODE_PARITY &&
                            nbmax + 1 > (double)min(lds_bins, EXACT_BINS);
     if (P.sorted && (big || big_exact)) {
@@ -1046,9 +1048,10 @@ __device__ __forceinline__ bool better(double s, int i, double bs, int bi) {
 // Running best window of the box scan.  A window's score is q/den with
 // q = num^2 (num = depth * W_in * W_out / W_tot^2) and den = w_in * w_out^2
 // (likelihood) or w_in * w_out (snr), a monotone transform of the oracle's
-// objective; scores are compared by cross-multiplication (no division),
-// equal scores keep the lowest (duration, start) index as the oracle's
-// first-wins loop does.
+// objective; a lane compares its windows by cross-multiplication (no
+// division), equal scores keep the lowest (duration, start) index as the
+// oracle's first-wins loop does; the lanes' winners are compared by the
+// quotient (block_argmax).
 struct Score {
     double q = -1.0, den = 1.0;
     int idx = INT_MAX;
@@ -1083,46 +1086,6 @@ struct Score {
             idx = i2;
         }
     }
-    __device__ __forceinline__ void wave_reduce() {
-        for (int o = 32; o > 0; o >>= 1) {
-            const double oq = __shfl_xor(q, o), od = __shfl_xor(den, o);
-            const int oi = __shfl_xor(idx, o);
-            offer(oq, od, oi);
-        }
-    }
-    // One step of wave_reduce_last: combine with the lane that DPP control
-    // CTRL names (lanes outside ROW_MASK, or reading past their row, keep
-    // their own value, and combining a score with itself changes nothing).
-    template <int CTRL, int ROW_MASK>
-    __device__ __forceinline__ void dpp_step() {
-        const int ql = __builtin_amdgcn_update_dpp(
-            __double2loint(q), __double2loint(q), CTRL, ROW_MASK, 0xF, false);
-        const int qh = __builtin_amdgcn_update_dpp(
-            __double2hiint(q), __double2hiint(q), CTRL, ROW_MASK, 0xF, false);
-        const int dl = __builtin_amdgcn_update_dpp(__double2loint(den),
-                                                   __double2loint(den), CTRL,
-                                                   ROW_MASK, 0xF, false);
-        const int dh = __builtin_amdgcn_update_dpp(__double2hiint(den),
-                                                   __double2hiint(den), CTRL,
-                                                   ROW_MASK, 0xF, false);
-        const int oi =
-            __builtin_amdgcn_update_dpp(idx, idx, CTRL, ROW_MASK, 0xF, false);
-        offer(__hiloint2double(qh, ql), __hiloint2double(dh, dl), oi);
-    }
-    // The wave's best into lane 63 with DPP lane moves (no LDS traffic):
-    // quad xor 1 and 2, half-row and row mirrors (each row of 16 then
-    // holds its best), then rows 1 and 3 take rows 0 and 2 (row_bcast:15)
-    // and rows 2 and 3 take row 1 (row_bcast:31).  offer() is commutative
-    // and associative (highest score, then lowest index), so the order of
-    // the combination does not change the result.
-    __device__ __forceinline__ void wave_reduce_last() {
-        dpp_step<0xB1, 0xF>();   // quad_perm [1,0,3,2]
-        dpp_step<0x4E, 0xF>();   // quad_perm [2,3,0,1]
-        dpp_step<0x141, 0xF>();  // row_half_mirror
-        dpp_step<0x140, 0xF>();  // row_mirror
-        dpp_step<0x142, 0xA>();  // row_bcast:15
-        dpp_step<0x143, 0xC>();  // row_bcast:31
-    }
 };
 
 __device__ __forceinline__ void write_nan(const Out &o, long long p) {
@@ -1145,9 +1108,29 @@ __device__ __forceinline__ double fold_mod(double x, double P) {
 // Sliding-box scan of one period's prefix arrays (Cw or the point counts
 // Cn, and Cy; index 0..nb).  Duration-major: a thread meets its windows in
 // increasing index (k * nval + s), so a later equal score never replaces
-// the held one.  The window tests are a predicate rather than branches, so
-// that the unrolled iterations issue all their LDS loads before the first
-// wait.
+// the held one.  Every lane runs the same number of iterations per duration
+// (the last one predicated off past the final start, its loads clamped to
+// it), so that a wave never runs a lane's remainder iterations on their own;
+// the window tests are a predicate rather than branches, so that the
+// unrolled iterations issue all their LDS loads before the first wait.
+// One window of the scan: start s, m bins, (duration, start) index idx;
+// counts only if in.
+template <bool PACKED, bool LIKE>
+__device__ __forceinline__ void box_window(Score &best, const double *Cw,
+                                           const int *Cn, const double *Cy,
+                                           double c1, double ytn, double epsn,
+                                           int s, int m, int idx, bool in) {
+    const double win =
+        PACKED ? (double)(Cn[s + m] - Cn[s]) * c1 : Cw[s + m] - Cw[s];
+    const double yin = Cy[s + m] - Cy[s];
+    const double wout = 1.0 - win;
+    // depth * W_in * W_out / W_tot^2 = y_tot*w_in - y_in
+    const double num = fma(ytn, win, -yin);
+    const bool ok = in & !(win < epsn) & !(wout < epsn) & !(num < 0.0);
+    best.offer_later(ok, num * num, LIKE ? win * wout * wout : win * wout,
+                     idx);
+}
+
 template <bool PACKED, bool LIKE>
 __device__ __forceinline__ void box_scan(Score &best, const double *Cw,
                                          const int *Cn, const double *Cy,
@@ -1157,20 +1140,134 @@ __device__ __forceinline__ void box_scan(Score &best, const double *Cw,
     for (int k = 0; k < nd; ++k) {
         const int m = s_m[k];
         const int send = min(nb - 1, nb - m);
+        // iterations in which every lane has a window, then one in which
+        // the lanes past the final start are predicated off (their loads
+        // clamped to it)
+        const int full = (send + 1) / NT;
+        const int kbase = k * nval;
+        const int s0 = (int)threadIdx.x;
 #pragma unroll 4
-        for (int s = threadIdx.x; s <= send; s += NT) {
-            const double win =
-                PACKED ? (double)(Cn[s + m] - Cn[s]) * c1 : Cw[s + m] - Cw[s];
-            const double yin = Cy[s + m] - Cy[s];
+        for (int i = 0; i < full; ++i) {
+            const int s = s0 + i * NT;
+            box_window<PACKED, LIKE>(best, Cw, Cn, Cy, c1, ytn, epsn, s, m,
+                                     kbase + s, true);
+        }
+        if (full * NT <= send) {
+            const int s = s0 + full * NT;
+            box_window<PACKED, LIKE>(best, Cw, Cn, Cy, c1, ytn, epsn,
+                                     min(s, send), m, kbase + s, s <= send);
+        }
+    }
+}
+
+// The box scan of k_main.  For duration m the starts 0..send (send =
+// nb - m) are split among the four waves in shares of q starts, q the
+// multiple of 64 with 4q >= send + 1; wave w takes the q starts from
+// min(w*q, send + 1 - q), so the shares cover every start, none runs past
+// send, and where they overlap a window is evaluated twice (the argmax
+// keeps one: same score, same index).  Every lane of a wave runs the same
+// q/64 strides, with no predicate or tail.  Only a duration with fewer than
+// 64 starts reads past its final window: PADN sentinel entries after the
+// period's prefix values make those windows fail the weight test (ending in
+// the pad they hold more than the whole weight, starting in it none).  The
+// score is num * |num| / den, so a window of negative depth never beats the
+// initial best (q = 0): only windows of positive depth are taken here, and
+// a period without one is rescanned with box_scan (the depth-0 windows the
+// oracle accepts).
+template <bool PACKED, bool LIKE>
+__device__ __forceinline__ void box_scan_pad(Score &best, const double *Cw,
+                                             const int *Cn, const double *Cy,
+                                             double c1, const short *s_m,
+                                             int nd, int nb, double ytn,
+                                             double epsn) {
+    const int nval = nb + 1;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int lane = (int)threadIdx.x & 63;
+    best.q = 0.0;
+    for (int k = 0; k < nd; ++k) {
+        const int m = s_m[k];
+        const int nst = nb - m + 1;  // starts 0..nb-m
+        if (nst <= 0) continue;
+        const int q = (nst + 4 * 64 - 1) / (4 * 64) * 64;
+        const int a = max(0, min(wv * q, nst - q));
+        const int s0 = a + lane;
+        const int *pn = Cn + s0, *pnm = Cn + s0 + m;
+        const double *pw = Cw + s0, *pwm = Cw + s0 + m;
+        const double *py = Cy + s0, *pym = Cy + s0 + m;
+        const int i0 = k * nval + s0;
+        const int trips = q / 64;
+#pragma unroll 4
+        for (int i = 0; i < trips; ++i) {
+            const int o = i * 64;
+            const double win = PACKED ? (double)(pnm[o] - pn[o]) * c1
+                                      : pwm[o] - pw[o];
+            const double yin = pym[o] - py[o];
             const double wout = 1.0 - win;
             // depth * W_in * W_out / W_tot^2 = y_tot*w_in - y_in
             const double num = fma(ytn, win, -yin);
-            const bool ok = !(win < epsn) & !(wout < epsn) & !(num < 0.0);
-            best.offer_later(ok, num * num,
-                             LIKE ? win * wout * wout : win * wout,
-                             k * nval + s);
+            best.offer_later(!(fmin(win, wout) < epsn), num * fabs(num),
+                             LIKE ? win * wout * wout : win * wout, i0 + o);
         }
     }
+}
+
+// Value of lane 63 of an FP64 register after a DPP max-reduction.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_d(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(
+        __double2loint(v), __double2loint(v), CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(
+        __double2hiint(v), __double2hiint(v), CTRL, ROW_MASK, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Workgroup argmax of the lanes' best windows: the highest score, then the
+// lowest index.  Each lane's (q, den) becomes one FP64 score q / den (-1
+// for a lane without a window); the wave's maximum goes to lane 63 with DPP
+// lane moves (quad swaps, half-row and row mirrors, then row broadcasts 15
+// and 31: every lane combines with lanes whose partial results cover the
+// rest of the wave), and then the lowest index among the lanes holding it
+// the same way.  Both operations are commutative and associative, so the
+// result does not depend on the order of combination.  Returns the index
+// (INT_MAX: no window) in every thread.
+__device__ __forceinline__ int block_argmax(const Score &best, double *dscr,
+                                            int *iscr) {
+    const double sc = best.idx == INT_MAX ? -1.0 : best.q / best.den;
+    double v = sc;
+    v = fmax(v, dpp_d<0xB1, 0xF>(v));   // quad_perm [1,0,3,2]
+    v = fmax(v, dpp_d<0x4E, 0xF>(v));   // quad_perm [2,3,0,1]
+    v = fmax(v, dpp_d<0x141, 0xF>(v));  // row_half_mirror
+    v = fmax(v, dpp_d<0x140, 0xF>(v));  // row_mirror
+    v = fmax(v, dpp_d<0x142, 0xA>(v));  // row_bcast:15
+    v = fmax(v, dpp_d<0x143, 0xC>(v));  // row_bcast:31
+    const double mx = readlane_d(v, 63);
+    int c = sc == mx ? best.idx : INT_MAX;
+    c = min(c, __builtin_amdgcn_update_dpp(c, c, 0xB1, 0xF, 0xF, false));
+    c = min(c, __builtin_amdgcn_update_dpp(c, c, 0x4E, 0xF, 0xF, false));
+    c = min(c, __builtin_amdgcn_update_dpp(c, c, 0x141, 0xF, 0xF, false));
+    c = min(c, __builtin_amdgcn_update_dpp(c, c, 0x140, 0xF, 0xF, false));
+    c = min(c, __builtin_amdgcn_update_dpp(c, c, 0x142, 0xA, 0xF, false));
+    c = min(c, __builtin_amdgcn_update_dpp(c, c, 0x143, 0xC, 0xF, false));
+    const int ci = __builtin_amdgcn_readlane(c, 63);
+    const int wv = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        dscr[wv] = mx;
+        iscr[wv] = ci;
+    }
+    __syncthreads();
+    double bm = dscr[0];
+    int bi = iscr[0];
+#pragma unroll
+    for (int i = 1; i < NW; ++i) {
+        const double d = dscr[i];
+        const int j = iscr[i];
+        if (d > bm || (d == bm && j < bi)) {
+            bm = d;
+            bi = j;
+        }
+    }
+    return bi;
 }
 
 // Padding, prefix scan and box scan of one period whose int64 histogram
@@ -1178,11 +1275,19 @@ __device__ __forceinline__ void box_scan(Score &best, const double *Cw,
 // packed histogram (P.packed) lives in binW alone (binY is unused): its
 // scan writes the FP64 flux prefix in place and the int32 point counts into
 // the (nb + 2) / 2 slots above it.
+// Entries past a period's last prefix value that its box scan reads: the
+// lanes past the final window of a duration with fewer than 64 starts reach
+// index 63 + m (m the duration's bins), so max(0, PADN + m_max - nb).
+__device__ __forceinline__ int scan_overhang(int nb, int mmax) {
+    return max(0, PADN + mmax - nb);
+}
+
+template <bool LIKE>
 __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
                               long long p, double period, int nb, int base,
                               long long *binW, long long *binY,
-                              const short *s_m, int nd, long long *lscr,
-                              double *dscr, int *iscr) {
+                              const short *s_m, int nd, int mmax,
+                              long long *lscr, double *dscr, int *iscr) {
     const int tid = threadIdx.x;
     const int os = b.oversample;
     const bool packed = P.packed;
@@ -1222,12 +1327,19 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
     double *Cw = (double *)binW + base;
     double *Cy = packed ? (double *)binW + base : (double *)binY + base;
     int *Cn = (int *)(binW + base + nval);
+    // packed prefix = (points << ybits) + flux sum, exactly; ybits >= 40,
+    // so the low word is the flux field's and the high word holds the
+    // count above its top hb = ybits - 32 (sign-extended) flux bits
+    const int hb = P.ybits - 32;
     for (int i = b0; i < b1; ++i) {
         ow += binW[base + i];
         if (packed) {
-            // the packed prefix holds (points << ybits) + flux sum exactly
-            const long long ys = packed_y(ow, P.ybits);
-            Cn[i] = (int)((ow - ys) >> P.ybits);
+            const int hi = (int)((unsigned long long)ow >> 32);
+            const int yhi = (int)((unsigned)hi << (32 - hb)) >> (32 - hb);
+            Cn[i] = (int)((unsigned)(hi - yhi) >> hb);
+            const long long ys =
+                (long long)(((unsigned long long)(unsigned)yhi << 32) |
+                            (unsigned)ow);
             Cy[i] = (double)ys * fy;
         } else {
             oy += binY[base + i];
@@ -1235,41 +1347,43 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
             Cy[i] = (double)oy * fy;
         }
     }
+    // the box scan's sentinels after the last prefix value: the total
+    // weight plus far more than all of it (the flux entries there are read
+    // but never decide a window)
+    const int over = scan_overhang(nb, mmax);
+    for (int t = tid; t < over; t += NT) {
+        if (packed) {
+            const int hi = (int)((unsigned long long)totw >> 32);
+            const int yhi = (int)((unsigned)hi << (32 - hb)) >> (32 - hb);
+            Cn[nval + t] = (int)((unsigned)(hi - yhi) >> hb) + (1 << 30);
+        } else {
+            Cw[nval + t] = (double)totw * fw + 4.0;
+        }
+    }
     __syncthreads();
 
     // box scan: start n, duration k (m bins), window (n, n+m]
     const double Wt = P.W_tot, Yt = P.Y_tot;
     const double ytn = Yt / Wt, epsn = DBL_EPSILON / Wt;
-    const bool like = b.objective == ATG_OBJ_LIKELIHOOD;
     Score best;
-    if (packed) {
-        if (like)
-            box_scan<true, true>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn, epsn);
-        else
-            box_scan<true, false>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn, epsn);
-    } else {
-        if (like)
-            box_scan<false, true>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn, epsn);
-        else
-            box_scan<false, false>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn,
-                                   epsn);
-    }
+    if (packed)
+        box_scan_pad<true, LIKE>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn, epsn);
+    else
+        box_scan_pad<false, LIKE>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn,
+                                  epsn);
     // workgroup argmax (highest score, then lowest (k, n))
-    best.wave_reduce_last();
-    const int wv = tid >> 6, l = tid & 63;
-    __syncthreads();
-    if (l == 63) {
-        dscr[wv] = best.q;
-        dscr[NW + wv] = best.den;
-        iscr[wv] = best.idx;
+    int bi = block_argmax(best, dscr, iscr);
+    if (bi == INT_MAX) {  // no window of positive depth: the oracle's choice
+                          // among those of depth 0, if any
+        Score zero;
+        if (packed)
+            box_scan<true, LIKE>(zero, Cw, Cn, Cy, c1, s_m, nd, nb, ytn, epsn);
+        else
+            box_scan<false, LIKE>(zero, Cw, Cn, Cy, c1, s_m, nd, nb, ytn,
+                                  epsn);
+        bi = block_argmax(zero, dscr, iscr);
     }
-    __syncthreads();
     if (tid == 0) {
-        best.q = dscr[0];
-        best.den = dscr[NW];
-        best.idx = iscr[0];
-        for (int i = 1; i < NW; ++i) best.offer(dscr[i], dscr[NW + i], iscr[i]);
-        const int bi = best.idx;
         if (bi == INT_MAX) {  // no window qualifies: power -inf as astropy
             o.power[p] = -INFINITY;
             o.depth[p] = 0.0;
@@ -1659,9 +1773,11 @@ __device__ __forceinline__ void flush_run(long long *binW, long long *binY,
 // Four waves per SIMD (at most 128 VGPRs): the fold, the LDS atomics and
 // the box scan's loads are latency-bound at three; four workgroups of
 // lds_bins <= 2528 share a CU's 160 KiB of LDS.
+// EXACT: parity mode's fold (fast mode's otherwise); LIKE: the likelihood
+// objective (the snr otherwise).
+template <bool EXACT, bool LIKE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4)))
-void k_main(Batch b, Out o, Work w,
-                                             int per_item, int lds_bins) {
+void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
     extern __shared__ __align__(16) unsigned char smem[];
     long long *binW = (long long *)smem;
     long long *binY = binW + lds_bins;
@@ -1731,6 +1847,9 @@ void k_main(Batch b, Out o, Work w,
         if (tid < nd)
             s_m[tid] = (short)fmin(round(b.durs[dof + tid] / P.bd), 32767.0);
         __syncthreads();
+        int mmax = 0;
+        for (int k = 0; k < nd; ++k) mmax = max(mmax, (int)s_m[k]);
+
         const double bd = P.bd;
         const double inv_bd = 1.0 / bd;
         const int os = b.oversample;
@@ -1758,13 +1877,18 @@ void k_main(Batch b, Out o, Work w,
                 if (cnt == s && p + s < pend) {
                     const double ps = b.periods[p + s];
                     const int nbs = (int)ceil(ps / bd) + os;
-                    if (used + (nbs + 1) + (packed ? (nbs + 2) / 2 : 0) <=
+                    // the period on top also needs room for its box-scan
+                    // sentinels: the overhang after its point counts
+                    // (packed) or weights (unpacked)
+                    const int ov = scan_overhang(nbs, mmax);
+                    if (used + (nbs + 1) +
+                            (packed ? max((nbs + 1 + ov + 1) / 2, ov) : ov) <=
                         cap) {
                         // the exact fold takes P/bd and 1/P scaled up by
                         // 1 + 2^-48 (see bin_points)
-                        const bool fastm = b.mode == ATG_MODE_FAST;
-                        per[s] = fastm ? ps : ps * inv_bd;
-                        inv[s] = fastm ? 1.0 / ps : (1.0 / ps) * (1.0 + 0x1p-48);
+                        per[s] = EXACT ? ps * inv_bd : ps;
+                        inv[s] = EXACT ? (1.0 / ps) * (1.0 + 0x1p-48)
+                                       : 1.0 / ps;
                         nb[s] = nbs;
                         base[s] = used;
                         used += nbs + 1;
@@ -1813,13 +1937,8 @@ void k_main(Batch b, Out o, Work w,
         ATG_BIN(5, E, K) ATG_BIN(6, E, K) ATG_BIN(7, E, K) ATG_BIN(8, E, K) \
         default: break;                                                     \
     }
-                if (b.mode == ATG_MODE_FAST) {
-                    if (packed) ATG_BIN8(false, true)
-                    else ATG_BIN8(false, false)
-                } else {
-                    if (packed) ATG_BIN8(true, true)
-                    else ATG_BIN8(true, false)
-                }
+                if (packed) ATG_BIN8(EXACT, true)
+                else ATG_BIN8(EXACT, false)
 #undef ATG_BIN8
 #undef ATG_BIN
             }
@@ -1831,18 +1950,21 @@ void k_main(Batch b, Out o, Work w,
                     const double ps = b.periods[p + s];
                     const int nbs = (int)ceil(ps / bd) + os;
                     const int bse = end - (nbs + 1);
-                    finish_period(b, o, P, p + s, ps, nbs, bse, binW, binW,
-                                  s_m, nd, lscr, dscr, iscr);
+                    finish_period<LIKE>(b, o, P, p + s, ps, nbs, bse, binW, binW,
+                                  s_m, nd, mmax, lscr, dscr, iscr);
                     end = bse;
                 }
             } else {
-                int bse = 0;
-                for (int s = 0; s < cnt; ++s) {
+                // last to first too: a period's sentinels overwrite only
+                // periods already finished
+                int end = used;
+                for (int s = cnt - 1; s >= 0; --s) {
                     const double ps = b.periods[p + s];
                     const int nbs = (int)ceil(ps / bd) + os;
-                    finish_period(b, o, P, p + s, ps, nbs, bse, binW, binY,
-                                  s_m, nd, lscr, dscr, iscr);
-                    bse += nbs + 1;
+                    const int bse = end - (nbs + 1);
+                    finish_period<LIKE>(b, o, P, p + s, ps, nbs, bse, binW, binY,
+                                  s_m, nd, mmax, lscr, dscr, iscr);
+                    end = bse;
                 }
             }
             p += cnt;
@@ -3297,7 +3419,10 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
             : (int)std::min<long long>(
                   24, std::max<long long>(4, b->n_periods / (8LL * grid)));
     if (lds > c->lds_attr_set) {
-        for (const void *f : {(const void *)k_main, (const void *)k_big}) {
+        for (const void *f :
+             {(const void *)k_main<true, true>, (const void *)k_main<true, false>,
+              (const void *)k_main<false, true>,
+              (const void *)k_main<false, false>, (const void *)k_big}) {
             e = hipFuncSetAttribute(
                 f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return hip_fail(c, e, "hipFuncSetAttribute");
@@ -3322,8 +3447,14 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
         }
         hipEventRecord(ev0, c->stream);
     }
-    hipLaunchKernelGGL(k_main, dim3(grid), dim3(NT), lds, c->stream, kb, ko,
-                       w, per_item, c->tune.lds_bins);
+    {
+        const bool ex = b->mode == ATG_MODE_PARITY;
+        const bool lk = b->objective == ATG_OBJ_LIKELIHOOD;
+        auto km = ex ? (lk ? k_main<true, true> : k_main<true, false>)
+                     : (lk ? k_main<false, true> : k_main<false, false>);
+        hipLaunchKernelGGL(km, dim3(grid), dim3(NT), lds, c->stream, kb, ko,
+                           w, per_item, c->tune.lds_bins);
+    }
     // k_big's histogram is 32 bins (512 B) smaller where that is what its
     // static LDS needs for as many workgroups per CU as k_main (its results
     // do not depend on the tile size)
