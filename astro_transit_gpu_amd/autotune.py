@@ -11,7 +11,11 @@ Candidates (each a legal launch on gfx950):
 Each candidate runs the main kernel on a sample of the workload and is timed
 with HIP events on the context's stream; the fastest is applied and cached
 per workload signature (curve count bucket, mean points, mean bins per
-period, durations) in a JSON file next to the library.
+period, durations, mode, objective, and whether the weights are uniform:
+uniform-weight curves take the packed 8-byte bins, the others twice the
+LDS per bin) in a JSON file next to the library.  Every tiling gives the
+same results bit for bit (integer bins; tested), so the choice only
+changes the time.
 """
 import json
 import os
@@ -29,7 +33,7 @@ STATIC_LDS = 256           # the main kernel's static LDS
 MAX_WAVES_PER_SIMD = 4     # register-limited
 # bumped whenever the kernel's speed profile changes, so that cached
 # tilings of an older build are not reused
-KERNEL_GENERATION = 4
+KERNEL_GENERATION = 5
 
 
 def candidates(lds_options=(2048, 2432, 2540, 3072, 4096, 5120, 6144),
@@ -45,13 +49,25 @@ def candidates(lds_options=(2048, 2432, 2540, 3072, 4096, 5120, 6144),
     return out
 
 
-def signature(n_lc, n_points_mean, bins_mean, n_dur, mode):
+def signature(n_lc, n_points_mean, bins_mean, n_dur, mode,
+              objective="likelihood", uniform_weights=True):
     """Workload key: buckets wide enough that nearby shapes share a
     tiling."""
-    return "g%d_lc%d_n%d_b%d_d%d_%s" % (
+    return "g%d_lc%d_n%d_b%d_d%d_%s_%s_%s" % (
         KERNEL_GENERATION,
         int(np.log2(max(n_lc, 1))), int(np.log2(max(n_points_mean, 1))),
-        int(np.log2(max(bins_mean, 1))), n_dur, mode)
+        int(np.log2(max(bins_mean, 1))), n_dur, mode, objective,
+        "uw" if uniform_weights else "vw")
+
+
+def uniform_weights(dys):
+    """True when every curve has one dy value (or none): the curves whose
+    bins the kernel packs (barring extreme outliers)."""
+    if dys is None:
+        return True
+    return all(d is None or np.size(d) == 0 or
+               np.all(np.asarray(d) == np.asarray(d).ravel()[0])
+               for d in dys)
 
 
 def _load():
@@ -83,7 +99,8 @@ def autotune(ctx: Context, ts, ys, dys, periods, durations,
     bd = np.min(dur) / oversample
     bins_mean = float(np.mean(np.ceil(np.asarray(per) / bd))) + oversample
     sig = signature(len(ts), np.mean([np.size(t) for t in ts]), bins_mean,
-                    np.size(dur), mode)
+                    np.size(dur), mode, objective,
+                    uniform_weights(None if dys is None else dys[:n]))
     cache = _load() if use_cache else {}
     if sig in cache:
         ctx.set_tuning(**cache[sig])

@@ -6,12 +6,16 @@ The reference's SectorCache (src/astrotransit_gpu/data/sector_cache.py:
 * "vectorized": one common time axis ``time`` (FP32) and ``flux`` /
   ``flux_err`` matrices of shape (targets, 1312) padded with
   ``flux_err = 1.0`` (constants.py:13-14); padded points are those with
-  ``flux_err >= 0.9`` (sector_cache.py:196-202), or
-* "flat": concatenated ``time`` / ``flux`` / ``flux_err`` with ``offsets``.
+  ``flux_err >= 0.9`` (sector_cache.py:193-200), or
+* "flat": concatenated ``time`` / ``flux`` / ``flux_err`` with ``offsets``
+  (one start per target, optionally a final end); every point of a target
+  is returned (sector_cache.py:201-210: no mask).
 
 ``load_sector_cache`` reads it without unpickling anything and returns the
-targets as a ragged host batch with the padding removed (the GPU path never
-sees padded points), ready for :meth:`DeviceBatch.from_host`.
+targets as a ragged host batch (FP64 times, exact widenings of the stored
+FP32), the vectorized padding removed, ready for
+:meth:`DeviceBatch.from_host`.  ``target_data`` is the reference's
+``SectorCache.get_target_data`` on that result.
 """
 import numpy as np
 
@@ -41,14 +45,27 @@ def load_sector_cache(path, pad_threshold=FLUX_ERR_PAD_THRESHOLD):
     else:
         if offsets is None:
             raise ValueError("flat sector cache without offsets")
-        ends = list(offsets[1:]) + [len(flux)] if len(offsets) == len(tic) \
-            else list(offsets[1:])
-        for a, b in zip(offsets[:len(ends)], ends):
-            keep = err[a:b] < pad_threshold
-            ts.append(time[a:b].astype(np.float64)[keep])
-            ys.append(flux[a:b][keep])
-            dys.append(err[a:b][keep])
+        # target i spans offsets[i] .. offsets[i+1] (the last one to the
+        # end when no final offset is stored), as get_target_data
+        for i in range(len(tic)):
+            a = int(offsets[i])
+            b = int(offsets[i + 1]) if i + 1 < len(offsets) else len(flux)
+            ts.append(time[a:b].astype(np.float64))
+            ys.append(flux[a:b])
+            dys.append(err[a:b])
     return dict(tic_ids=tic, t=ts, y=ys, dy=dys)
+
+
+def target_data(loaded, tic_id):
+    """time / flux / flux_err of the first target with this TIC id, or None
+    (the reference's SectorCache.get_target_data,
+    data/sector_cache.py:174-210, on load_sector_cache's result)."""
+    idx = np.nonzero(np.asarray(loaded["tic_ids"]) == int(tic_id))[0]
+    if idx.size == 0:
+        return None
+    i = int(idx[0])
+    return {"time": loaded["t"][i], "flux": loaded["y"][i],
+            "flux_err": loaded["dy"][i]}
 
 
 def write_sector_cache(path, time, flux, flux_err, tic_ids, length=1312):

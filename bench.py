@@ -69,12 +69,27 @@ def measured_traffic(n_lc):
         return None
 
 
-def cpu_baseline(d, n_sample, objective, gpu=None):
+def host_threads():
+    """Host threads this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS where that is set (a shared GPU host gives each GPU's
+    process a share of its cores through it)."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        n = min(n, int(os.environ.get("OMP_NUM_THREADS", n)))
+    except ValueError:
+        pass
+    return max(1, n)
+
+
+def cpu_baseline(d, n_sample, objective, gpu=None, n_single=8):
     """The FP64 C oracle on the host cores over the first ``n_sample``
-    curves.  With ``gpu`` (host copies of the timed run's outputs and period
-    offsets) the same oracle outputs give the parity of those curves."""
+    curves (OpenMP over periods on every thread this process may use), and
+    single-threaded over the first ``n_single`` curves (the like of the
+    reference's single-process astropy figure).  With ``gpu`` (host copies
+    of the timed run's outputs and period offsets) the same oracle outputs
+    give the parity of those curves."""
     from oracle import oracle
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores = host_threads()
     refs = []
     s = time.perf_counter()
     for i in range(n_sample):
@@ -82,9 +97,19 @@ def cpu_baseline(d, n_sample, objective, gpu=None):
                                  d["periods"], d["durations"], objective,
                                  n_threads=cores))
     el = time.perf_counter() - s
+    s = time.perf_counter()
+    for i in range(n_single):
+        oracle.power(d["t"][i], d["y"][i], d["dy"][i], d["periods"],
+                     d["durations"], objective, n_threads=1)
+    el1 = time.perf_counter() - s
     out = dict(value=n_sample / el, unit="LC/s", cores=cores, kind="port",
                sample="%d of the 1,000 curves, full 5,000x5 grid, FP64 C "
-               "oracle (OpenMP over periods), %.1f s" % (n_sample, el))
+               "oracle (OpenMP over periods, %d threads), %.1f s" % (
+                   n_sample, cores, el),
+               single_core={"value": n_single / el1, "unit": "LC/s",
+                            "cores": 1,
+                            "sample": "%d of the 1,000 curves, one thread, "
+                            "%.1f s" % (n_single, el1)})
     if gpu is None:
         return out, None
     h, po = gpu
@@ -135,7 +160,30 @@ def ragged_leg(n_lc, objective):
     el = time.perf_counter() - s
     folds = sum(np.size(t) * np.size(p) for t, p in zip(d["t"], d["periods"]))
     st = out.status.cpu().numpy()
+    # the replay's results on the first n_check curves against the FP64
+    # oracle (the tiling changes the time only)
+    from oracle import oracle
+    h = out.to_host()
+    po = b.host_period_offsets
+    n_check = min(32, n_lc)
+    rel_max, same, exact = 0.0, 0, 0
+    for i in range(n_check):
+        r = oracle.power(d["t"][i], d["y"][i], d["dy"][i], d["periods"][i],
+                         d["durations"][i], objective,
+                         n_threads=host_threads())
+        g = {k: h[k][po[i]:po[i + 1]] for k in r}
+        fin = np.isfinite(r["power"]) & (r["power"] != 0)
+        rel_max = max(rel_max, float(np.max(
+            np.abs(g["power"][fin] / r["power"][fin] - 1.0))))
+        ib = int(np.argmax(r["power"]))
+        same += int(int(h["best_index"][i]) == ib)
+        exact += int(all(g[k][ib] == r[k][ib] for k in r))
     return {"value": round(n_lc / el, 1), "unit": "LC/s", "curves": n_lc,
+            "parity_vs_oracle": {
+                "curves": n_check, "power_rel_err_max": float(
+                    "%.3g" % rel_max),
+                "best_period_same": "%d/%d" % (same, n_check),
+                "best_fields_bit_exact": "%d/%d" % (exact, n_check)},
             "points": int(sum(np.size(t) for t in d["t"])),
             "periods": int(sum(np.size(p) for p in d["periods"])),
             "folds_per_s": float("%.4g" % (folds / el)),
@@ -254,7 +302,7 @@ def precision_leg(n_lc, n_sample, objective, seconds_cap=None):
         print("precision: GPU %s %.1f s" % (mode, el), file=sys.stderr,
               flush=True)
         del res, h
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores = host_threads()
     stats = {k: [] for k in ("gpu_parity_f64fold_i64bins",
                              "gpu_fast_f64fmafold_i64bins",
                              "fp32_fold", "fp32_bins_and_scan",
@@ -301,7 +349,7 @@ def parity_sweep(n_lc, objective):
     import torch
     from oracle import oracle
     from astro_transit_gpu_amd import Context, DeviceBatch, synth
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores = host_threads()
     ctx = Context(0)
     out = {}
     for name, d in (("1000x15k_5000p_5d", synth.config2(n_lc=n_lc, seed=2)),

@@ -139,5 +139,13 @@ def test_refine_targets_equals_per_target_power():
         assert got == want, tid
         assert all(r["status"] == "ok" and r["n_points"] == t.size
                    for r in rows if r["tic_id"] == tid)
+        # anchored on the FP64 oracle (snr objective, the API's default):
+        # the first row is an oracle maximum within the fast mode's drift
+        # and every row's power is the oracle's at its period within 1e-6
+        ref = O.power(t, y, dy, periods, durs, "snr")["power"]
+        for r in (x for x in rows if x["tic_id"] == tid):
+            i = int(np.nonzero(periods == r["period"])[0][0])
+            assert abs(r["power"] - ref[i]) <= 1e-6 * abs(ref[i]), (tid, i)
+        assert got[0][4] >= ref.max() * (1 - 1e-6), tid
     bad = [r for r in rows if r["tic_id"] == 4004]
     assert len(bad) == 1 and bad[0]["status"] == "error" and bad[0]["error"]

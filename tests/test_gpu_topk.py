@@ -98,6 +98,18 @@ def test_api_top_candidates(ctx):
         assert (a.period, a.t0, a.duration, a.depth, a.power) == (
             b["period"], b["t0"], b["duration"], b["depth"], b["power"])
     assert r.top_candidates[0].period == r.best_period
+    # anchored on astropy's periodogram of the same curve: the host rule on
+    # the golden power array picks the same five periods, and each
+    # candidate's power is astropy's at that period within the parity bar
+    ref = c["out"]
+    gold = get_top_k_candidates(dict(power=ref["power"], periods=c["periods"],
+                                     all_t0s=ref["transit_time"],
+                                     all_durs=ref["duration"],
+                                     all_depths=ref["depth"]), k=5)
+    assert [g["period"] for g in gold] == [a.period for a in r.top_candidates]
+    for a in r.top_candidates:
+        i = int(np.nonzero(c["periods"] == a.period)[0][0])
+        assert abs(a.power - ref["power"][i]) <= 1e-6 * abs(ref["power"][i])
 
 
 def test_candidates_batch(ctx):
