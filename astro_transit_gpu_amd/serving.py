@@ -109,11 +109,14 @@ class BLSServer:
                  oversample=10, hbm_budget_bytes=None, capacity_points=None,
                  capacity_periods=None, capacity_durations=None,
                  typical_n=15000, typical_p=5000, max_batch=1024, n_slots=2,
-                 y_dtype=np.float32, tuning=None):
+                 y_dtype=np.float32, tuning=None, keep_results=True):
         self.device = torch.device(device if device is not None else "cuda")
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
         self.objective, self.mode, self.oversample = objective, mode, oversample
+        # keep_results=False: a request keeps its best index and status
+        # only, so that a long stream does not hold every periodogram
+        self.keep_results = keep_results
         self.max_batch = max_batch
         self.y_dtype = np.dtype(y_dtype)
         ytd = torch.float32 if self.y_dtype == np.float32 else torch.float64
@@ -303,7 +306,8 @@ class BLSServer:
         h = {k: v.numpy() for k, v in b["host"].items()}
         for i, r in enumerate(b["reqs"]):
             # views of the batch's pinned result buffers (kept alive by them)
-            r.result = {k: h[k][po[i]:po[i + 1]] for k in FIELDS}
+            if self.keep_results:
+                r.result = {k: h[k][po[i]:po[i + 1]] for k in FIELDS}
             r.best_index = int(b["bi"][i])
             r.status = int(b["st"][i])
             r.done = now
