@@ -19,7 +19,7 @@ LC_STATUS = {
     0: "ok", 1: "invalid period", 2: "invalid duration", 3: "NaN in t or y",
     4: "dy <= 0 or NaN", 5: "period needs more bins than LDS holds",
     6: "empty light curve or grid", 7: "more than 64 durations",
-    8: "more than 2^30 points in one light curve",
+    8: "more than 2^28 points in one light curve",
 }
 
 ATG_F32 = 1

@@ -354,6 +354,21 @@ __device__ int lower_key(const double *tn, int n, double P, double invP,
     return lo;
 }
 
+// A workgroup-uniform global pointer moved to scalar registers, and loads
+// from it at a 32-bit byte offset (the base + offset addressing of global
+// loads, with no 64-bit address per lane).
+typedef __attribute__((address_space(1))) const char gchar;
+__device__ __forceinline__ gchar *scalar_base(const void *p) {
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (gchar *)(((unsigned long long)hi << 32) | lo);
+}
+template <typename T>
+__device__ __forceinline__ T load_at(gchar *base, unsigned byte_off) {
+    return *(const __attribute__((address_space(1))) T *)(base + byte_off);
+}
+
 // Bins every point of the lane for C periods at once.  C is a compile-time
 // constant so that the C fold chains are independent straight-line code the
 // scheduler can interleave; the next point's loads are issued before the
@@ -369,12 +384,15 @@ __device__ __forceinline__ void bin_points(
     const double *per, const double *inv, const int *nb, const int *base,
     double bd, double inv_bd, double tol_x, long long *binW,
     long long *binY, const double *pg) {
-    // tp, wp, yp: the curve's (uniform) slot bases; lane tid reads slots
-    // tid + i * NT through a 32-bit offset, so that the loads address
-    // base + offset and no 64-bit per-lane pointer stays live
-    unsigned int sl = threadIdx.x;
-    double tv_n = tp[sl];
-    long long wv_n = wp[sl], yv_n = PACKED ? 0 : yp[sl];
+    // tp, wp, yp: the curve's (uniform) slot bases, in scalar registers;
+    // lane tid loads slot tid + i * NT at a 32-bit byte offset from them
+    // (base + offset addressing: no per-lane 64-bit address arithmetic; a
+    // curve of at most 2^28 points spans less than 2^32 bytes)
+    gchar *bt = scalar_base(tp), *bw = scalar_base(wp), *by = scalar_base(yp);
+    unsigned int lo = threadIdx.x * 8u;
+    double tv_n = load_at<double>(bt, lo);
+    long long wv_n = load_at<long long>(bw, lo);
+    long long yv_n = PACKED ? 0 : load_at<long long>(by, lo);
     const double tol2 = 2.0 * tol_x;
     // per-period bin origins (bin 1 for the exact fold, bin 0 for the fast
     // one), so that a bin's LDS address is one shift-and-add of its index
@@ -391,16 +409,17 @@ __device__ __forceinline__ void bin_points(
         const double tv = tv_n;
         const long long wv = wv_n, yv = yv_n;
         if (i + 1 < L) {
-            sl += NT;
-            tv_n = tp[sl];
-            wv_n = wp[sl];
-            if (!PACKED) yv_n = yp[sl];
+            lo += NT * 8;
+            tv_n = load_at<double>(bt, lo);
+            wv_n = load_at<long long>(bw, lo);
+            if (!PACKED) yv_n = load_at<long long>(by, lo);
         }
         int at[C];
         bool fl[C];
         bool any = false;
-        // t'/bd + tol_x, shared by the C periods of the pass
-        const double tb = EXACT ? tv * inv_bd + tol_x : 0.0;
+        // t'/bd + tol_x, shared by the C periods of the pass (one rounding;
+        // within the tolerance analysis below, which allows two)
+        const double tb = EXACT ? fma(tv, inv_bd, tol_x) : 0.0;
 #pragma unroll
         for (int s = 0; s < C; ++s) {
             if (EXACT) {
@@ -461,7 +480,7 @@ __device__ __forceinline__ void bin_points(
                     const double Ps = pgx[s];
                     const int a = base[s] + 1 +
                                   (int)fma(floor(tx * inv[s]), -per[s],
-                                           tx * inv_bd + tol_x);
+                                           fma(tx, inv_bd, tol_x));
                     const double ph = tx - Ps * floor(tx / Ps);
                     const int e =
                         base[s] + min(max((int)trunc(ph / bd) + 1, 0), nb[s]);
@@ -598,9 +617,10 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     const long long n64 = b.lc_off[lc + 1] - off;
     const long long po = b.per_off[lc], np = b.per_off[lc + 1] - po;
     const long long dof = b.dur_off[lc], nd = b.dur_off[lc + 1] - dof;
-    // point indices within a curve are 32-bit (with headroom for the
-    // padded per-curve slots)
-    const bool too_long = n64 > ((long long)1 << 30);
+    // point indices within a curve are 32-bit, and k_main addresses a
+    // curve's lane-major slots (8 B each) by 32-bit byte offsets: at most
+    // 2^28 points per curve
+    const bool too_long = n64 > ((long long)1 << 28);
     const int n = too_long ? 0 : (int)n64;
     const int tid = threadIdx.x;
     if (tid == 0) w.lc_flags[lc] = 0;  // workspace bytes are reused across shapes
@@ -1442,21 +1462,6 @@ __device__ __forceinline__ void seg_add(int j, int s0, int hi, long long wv,
 // that four blocks' loads are issued before the first point is folded
 // (points of zero weight add zero); every loaded block starts below e and
 // so lies inside the curve's padded copy.
-// A workgroup-uniform global pointer moved to scalar registers, and loads
-// from it at a 32-bit byte offset (the base + offset addressing of global
-// loads, with no 64-bit address per lane).
-typedef __attribute__((address_space(1))) const char gchar;
-__device__ __forceinline__ gchar *scalar_base(const void *p) {
-    const unsigned long long v = (unsigned long long)p;
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
-    return (gchar *)(((unsigned long long)hi << 32) | lo);
-}
-template <typename T>
-__device__ __forceinline__ T load_at(gchar *base, unsigned byte_off) {
-    return *(const __attribute__((address_space(1))) T *)(base + byte_off);
-}
-
 // (the curve's padded copy must span less than 2^32 bytes)
 template <bool FAST, bool PAD>
 __device__ __forceinline__ void big_fold_range(

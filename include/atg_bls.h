@@ -52,7 +52,7 @@ extern "C" {
                                   (or 2^30 bins)                          */
 #define ATG_LC_EMPTY 6        /* no points, periods or durations           */
 #define ATG_LC_TOO_MANY_DURATIONS 7 /* more than 64 durations             */
-#define ATG_LC_TOO_MANY_POINTS 8 /* more than 2^30 points in one curve     */
+#define ATG_LC_TOO_MANY_POINTS 8 /* more than 2^28 points in one curve     */
 
 /* element types */
 #define ATG_F32 1
