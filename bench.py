@@ -503,14 +503,15 @@ def main():
                                   d["durations"], device=ctx.device)
     out = ctx.run(batch, objective=objective, mode="parity")
 
-    def timed(mode, steps, warmup):
+    def timed(mode, steps, warmup, bt=None):
+        bt = batch if bt is None else bt
         for _ in range(warmup):
-            ctx.run(batch, objective=objective, mode=mode, out=out, sync=False)
+            ctx.run(bt, objective=objective, mode=mode, out=out, sync=False)
         torch.cuda.synchronize()
         ctx.timing(True, True)
         s = time.perf_counter()
         for _ in range(steps):
-            ctx.run(batch, objective=objective, mode=mode, out=out, sync=False)
+            ctx.run(bt, objective=objective, mode=mode, out=out, sync=False)
         torch.cuda.synchronize()
         el = time.perf_counter() - s
         n, kms = ctx.timing(False, True)
@@ -568,6 +569,22 @@ def main():
         line["fast"] = {"value": round(args.n_lc * max(2, args.steps // 2) /
                                        el_f, 2), "unit": "LC/s",
                         "kernel_ms": round(kms_f, 3)}
+        # the same workload with per-point dy (each point's dy scaled by a
+        # seeded U(0.9, 1.1)): unpacked bins, two int64 LDS adds per point
+        # and period
+        rng = np.random.default_rng(22)
+        dyh = [(dd * rng.uniform(0.9, 1.1, dd.size)).astype(np.float32)
+               for dd in d["dy"]]
+        bh = DeviceBatch.from_host(d["t"], d["y"], dyh, d["periods"],
+                                   d["durations"], device=ctx.device)
+        nh = max(2, args.steps // 2)
+        el_h, kms_h = timed("parity", nh, 1, bh)
+        line["hetero_dy"] = {"value": round(args.n_lc * nh / el_h, 2),
+                             "unit": "LC/s", "kernel_ms": round(kms_h, 3),
+                             "mode": "parity",
+                             "over_headline_time": round(
+                                 (el_h / nh) / (el / args.steps), 3)}
+        del bh
         # top-5 candidates of every curve: device top-k vs the reference's
         # host rule (scipy find_peaks over each periodogram)
         from astro_transit_gpu_amd import get_top_k_candidates
