@@ -54,18 +54,60 @@ def algorithmic_work(n_points, periods, durations, oversample=10):
                 sum_bins=int(nb.sum()))
 
 
+def profile_block(name):
+    """The ``key=value`` lines of a committed rocprofv3 summary under
+    profiles/ (its machine-readable block), as floats; None if absent."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        kv = {}
+        for line in open(path):
+            k, sep, v = line.strip().partition("=")
+            if sep and k and " " not in k:
+                try:
+                    kv[k] = float(v)
+                except ValueError:
+                    kv[k] = v
+        return kv or None
+    except OSError:
+        return None
+
+
 def measured_traffic(n_lc):
     """HBM bytes per launch of the periodogram stage (k_main, k_big and
     k_fields) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
     (copy-kernel calibrated), scaled to n_lc curves."""
-    path = os.path.join(ROOT, "profiles",
-                        "periodogram_stage_hbm_traffic_1000x15k_5000p_5d.md")
+    kv = profile_block("periodogram_stage_hbm_traffic_1000x15k_5000p_5d.md")
     try:
-        kv = dict(line.strip().split("=", 1) for line in open(path)
-                  if "=" in line and line.strip().split("=", 1)[0] in
-                  ("n_lc", "traffic_bytes"))
-        return int(int(kv["traffic_bytes"]) * n_lc / int(kv["n_lc"]))
-    except (OSError, KeyError, ValueError):
+        return int(kv["traffic_bytes"] * n_lc / kv["n_lc"])
+    except (TypeError, KeyError):
+        return None
+
+
+def measured_rates(n_lc, kernel_ms, algo_valu):
+    """Counter-measured VALU and LDS rates of k_main at the headline shape:
+    the per-launch counts of the committed rocprofv3 summary (SQ_INSTS_VALU,
+    SQ_ACTIVE_INST_VALU, the LDS _BANDWIDTH counters, SQ_LDS_BANK_CONFLICT),
+    scaled to n_lc curves, over this run's live stage time ``kernel_ms``."""
+    name = "k_main_counters_1000x15k_5000p_5d.md"
+    kv = profile_block(name)
+    try:
+        f = n_lc / kv["n_lc"]
+        lane_ops = kv["valu_insts"] * 64.0 * f
+        lds = kv["lds_bytes"] * f
+        sec = kernel_ms * 1e-3
+        return {
+            "source": "profiles/" + name,
+            "valu_lane_ops_per_launch": lane_ops,
+            "valu_achieved": round(lane_ops / sec / 1e12, 3),
+            "valu_frac": round(lane_ops / sec / PEAK_VALU, 4),
+            "valu_busy": round(kv["valu_busy"], 4),
+            "algorithmic_share_of_issued": round(algo_valu / lane_ops, 4),
+            "lds_bytes_per_launch": lds,
+            "lds_achieved_GBps": round(lds / sec / 1e9, 1),
+            "lds_frac": round(lds / sec / PEAK_LDS, 4),
+            "lds_bank_conflict_frac": round(kv["lds_bank_conflict_frac"], 4),
+            "profiled_kernel_ms": kv["duration_ms"]}
+    except (TypeError, KeyError, ZeroDivisionError):
         return None
 
 
@@ -322,8 +364,23 @@ def precision_leg(n_lc, n_sample, objective, seconds_cap=None):
         out[mode] = [{k: v.copy() for k, v in res.curve(i, h).items()}
                      for i in range(n_sample)]
         gpu[mode] = {"LC_per_s": round(n_lc / el, 2), "seconds": round(el, 2),
-                     "k_main_ms": round(kms, 1),
+                     "stage_ms": round(kms, 1),
                      "statuses_ok": int((h["status"] == 0).sum())}
+        # k_big's rocprofv3 counters at this shape (64 curves; rates are
+        # per launch and do not depend on the curve count)
+        kv = profile_block("k_big_counters_64x200k_50000p_6d_%s.md" % mode)
+        if kv:
+            gpu[mode]["k_big_counters"] = {
+                "source": "profiles/k_big_counters_64x200k_50000p_6d_%s.md"
+                % mode,
+                "valu_issue_frac": round(kv["valu_insts"] * 64.0 /
+                                         (kv["duration_ms"] * 1e-3) /
+                                         PEAK_VALU, 4),
+                "valu_busy": round(kv["valu_busy"], 4),
+                "lds_frac": round(kv["lds_bytes"] /
+                                  (kv["duration_ms"] * 1e-3) / PEAK_LDS, 4),
+                "lds_bank_conflict_frac": round(
+                    kv["lds_bank_conflict_frac"], 4)}
         print("precision: GPU %s %.1f s" % (mode, el), file=sys.stderr,
               flush=True)
         del res, h
@@ -553,6 +610,9 @@ def main():
                     "peak_GBps": PEAK_LDS / 1e9,
                     "frac": round(ach_lds / PEAK_LDS, 4),
                     "bytes_per_launch": per_launch_lds},
+            # achieved/frac above count the algorithmic work; "measured"
+            # counts what the hardware issued (rocprofv3 counters)
+            "measured": measured_rates(args.n_lc, kms, per_launch_valu),
             # the algorithmic minimum (inputs 16 B/point, outputs) and what
             # the stage moves once by design: the packed lane-major points
             # (16 B/point; every period of this workload fits the LDS
