@@ -1439,72 +1439,100 @@ __device__ __forceinline__ void pad_targets(int r, int nb, int os, int &j1,
     j2 = (r >= 1 && r <= os) ? r + nb - os - 1 : -1;
 }
 
+// One point into an interleaved bin: weight at bin[0], weight*flux at
+// bin[1] (one address, two atomics at offsets 0 and 8).
+__device__ __forceinline__ void bin_add(long long *bin, long long wv,
+                                        long long yv) {
+    __hip_atomic_fetch_add(bin, wv, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(bin + 1, yv, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __device__ __forceinline__ void seg_add(int j, int s0, int hi, long long wv,
                                         long long yv, long long &cw,
-                                        long long &cy, long long *binW,
-                                        long long *binY) {
+                                        long long &cy, long long *bins) {
     if (j < 0) return;
     if (j < s0) {
         cw += wv;
         cy += yv;
     } else if (j <= hi) {
-        __hip_atomic_fetch_add(&binW[j - s0], wv, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(&binY[j - s0], yv, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        bin_add(bins + 2 * (j - s0), wv, yv);
     }
 }
 
 // Ranged fold of one cycle's point range [a, e) of a time-sorted curve
-// into the LDS segment of bins [s0, hi], one block of the natural copy per
-// step: thread tid takes slot tid, which holds point nat_slot^-1(tid) of the
-// block.  PAD selects the destination: the point's own bin r (unless the
-// padding assignment overwrites it), or, for r in 1..os, the padding copy
-// r + nb - os - 1.  Points outside [a, e) are masked at the atomic, so
-// that four blocks' loads are issued before the first point is folded
-// (points of zero weight add zero); every loaded block starts below e and
-// so lies inside the curve's padded copy.
+// into the LDS segment of bins [s0, hi] (interleaved: bin s0 + l at
+// bins[2l], bins[2l + 1]), one block of the point-order copy per step:
+// thread tid takes slot tid, which holds point nat_slot^-1(tid) of the
+// block.  The range comes from lower_key with the fold of this mode, so
+// every point in it has its bin r (the oracle's in parity mode) in
+// [s0, hi]: the blocks wholly inside [a, e) are folded with no per-point
+// test, four blocks' loads issued before the first point is folded; the
+// partial blocks at either end test the point index.  PAD sends a point of
+// bin r (1..os) to its padding copy r + nb - os - 1 instead, which may lie
+// outside the segment (then it is skipped).  A direct point whose own bin
+// is one the padding assignment overwrites (nb - os .. nb - 1) is added
+// like the others; big_period clears those bins before the copies go in.
 // (the curve's padded copy must span less than 2^32 bytes)
 // UW: every point of the curve has the weight wone, which is then not
 // loaded (16 instead of 24 bytes read per point and period).
 template <bool FAST, bool PAD, bool UW>
 __device__ __forceinline__ void big_fold_range(
     const double *tn, const long long *wn, const long long *yn, int a, int e,
-    double period, double qc, double bd, double inv_bd, int nb, int os,
-    int s0, int hi, long long *binW, long long *binY, long long wone) {
+    double period, double qc, double bd, double inv_bd, double tol_x, int nb,
+    int os, int s0, int hi, long long *bins, long long wone) {
+    if (a >= e) return;
     const unsigned tid = threadIdx.x;
     const int pt = (tid & 63) * NW + (tid >> 6);  // point of slot tid
     gchar *tb = scalar_base(tn), *wb = scalar_base(wn), *yb = scalar_base(yn);
     const unsigned span = (unsigned)(hi - s0), cnt = (unsigned)(e - a);
-    // exact fold with one uniform tolerance: x = ph/bd lies in [-1, nb + 1]
-    // and the reciprocal product (with the shift) is within
-    // (nb + 2) * 2^-51 of it, so a point whose shifted product xs keeps
-    // more than 2*tol above its integer part has trunc(xs) == trunc(ph/bd);
-    // the others (a 2*tol share of them) take the division of fold_bin_in
-    const double tol = ldexp((double)nb + 2.0, -48), pq = period * qc;
-    auto bin_of = [&](double t) -> int {
-        if (FAST) return fold_bin_in(t, period, qc, bd, inv_bd, nb, true);
-        const double ph = t - pq;
-        const double xs = fma(ph, inv_bd, tol);
-        const double xi = trunc(xs);
-        if (xs - xi <= 2.0 * tol)
-            return min(max((int)trunc(ph / bd) + 1, 0), nb);
-        return min((int)xi + 1, nb);
-    };
-    auto add = [&](int j, double t, long long wv, long long yv) {
-        const int r = bin_of(t);
-        const int d = PAD ? r + nb - os - 1 : r;
-        const bool own = PAD ? (r >= 1 && r <= os) : (r < nb - os || r == nb);
-        const unsigned l = (unsigned)(d - s0);
-        if (own & (l <= span) & ((unsigned)(j - a) < cnt)) {
-            __hip_atomic_fetch_add(&binW[l], wv, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(&binY[l], yv, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+    const double pq = period * qc;
+    // Parity mode: x = t * inv_bd + (tol_x - pq * inv_bd), one FMA, is within
+    // u*(4*t/bd + 2*nb + 2) (u = 2^-53: the reciprocal, the product pq *
+    // inv_bd, the shift and the FMA) of the oracle's (t - pq)/bd + tol_x;
+    // tol_x (2^-47 times the curve's span plus its longest period, in bins)
+    // exceeds that 16 times.  So an x whose fraction exceeds 2*tol_x lies in
+    // the same open unit interval as the oracle's quotient and truncates to
+    // its bin; the others take the oracle's division.
+    const double off = tol_x - pq * inv_bd, tol2 = 2.0 * tol_x;
+    // destination of bin r, relative to s0
+    const int dsh = PAD ? nb - os - 1 - s0 : -s0;
+    auto rough = [&](double t, bool &flag) -> int {
+        if (FAST) {  // lower_key's fold, clamps included
+            flag = false;
+            return fold_bin_in(t, period, qc, bd, inv_bd, nb, true);
         }
+        const double x = fma(t, inv_bd, off);
+        flag = __builtin_amdgcn_fract(x) <= tol2;
+        return (int)x + 1;
     };
-    int B = a & ~(NT - 1);
-    for (; B + 3 * NT < e; B += 4 * NT) {
+    auto exact = [&](double t) -> int {
+        return min(max((int)trunc((t - pq) / bd) + 1, 0), nb);
+    };
+    // masked: test the point index (partial blocks); PAD also the segment
+    auto fold_block = [&](int B, bool masked) {
+        const unsigned o = ((unsigned)B + tid) * 8u;
+        const double t = load_at<double>(tb, o);
+        const long long wv = UW ? wone : load_at<long long>(wb, o);
+        const long long yv = load_at<long long>(yb, o);
+        bool fl;
+        int r = rough(t, fl);
+        if (fl) r = exact(t);
+        const unsigned l = (unsigned)(r + dsh);
+        const bool in = (!masked || (unsigned)(B + pt - a) < cnt) &&
+                        (!PAD || l <= span);
+        if (in) bin_add(bins + 2 * min(l, span), wv, yv);
+    };
+    const int Bf = (a + NT - 1) & ~(NT - 1);  // first whole block
+    const int Be = e & ~(NT - 1);             // end of the whole blocks
+    if (Bf > Be) {  // [a, e) within one block
+        fold_block(a & ~(NT - 1), true);
+        return;
+    }
+    if (a < Bf) fold_block(Bf - NT, true);
+    int B = Bf;
+    for (; B + 3 * NT < Be; B += 4 * NT) {
         const unsigned o = ((unsigned)B + tid) * 8u;
         double t[4];
         long long wv[4], yv[4];
@@ -1515,19 +1543,24 @@ __device__ __forceinline__ void big_fold_range(
             yv[u] = load_at<long long>(yb, o + u * NT * 8);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) add(B + u * NT + pt, t[u], wv[u], yv[u]);
+        for (int u = 0; u < 4; ++u) {
+            bool fl;
+            int r = rough(t[u], fl);
+            if (fl) r = exact(t[u]);
+            const unsigned l = (unsigned)(r + dsh);
+            if (!PAD || l <= span)
+                bin_add(bins + 2 * min(l, span), wv[u], yv[u]);
+        }
     }
-    for (; B < e; B += NT) {
-        const unsigned o = ((unsigned)B + tid) * 8u;
-        add(B + pt, load_at<double>(tb, o),
-            UW ? wone : load_at<long long>(wb, o), load_at<long long>(yb, o));
-    }
+    for (; B < Be; B += NT) fold_block(B, false);
+    if (Be < e) fold_block(Be, true);
 }
 
 // One period whose n_bins + 1 exceeds the LDS histogram: the start
 // positions are processed in segments of S start positions; segment k keeps
-// bins [s0, s0 + S + mmax] in LDS on top of the exact int64 sum of all bins
-// below s0 (the carried prefix).
+// bins [s0, s0 + S + mmax] in LDS (interleaved weight and weight*flux
+// sums, 16 B per bin) on top of the exact int64 sum of all bins below s0
+// (the carried prefix).
 //
 // Ranged form (time-sorted curves): the points of cycle c whose bin lies in
 // [s0, hi] are the index range [lower_key(c, s0), lower_key(c, hi + 1)), so
@@ -1538,10 +1571,10 @@ __device__ __forceinline__ void big_fold_range(
 // sums the ones below s0.
 __device__ void big_period(const Batch &b, const Out &o, const Work &w,
                            const LcParams &P, long long p, int nb,
-                           int lds_bins, long long *binW, long long *binY,
-                           const int *s_m, int nd, int mmax, long long *lscr,
-                           double *dscr, int *iscr, double *dscr2,
-                           double *dscr3, long long *s_carry) {
+                           int lds_bins, long long *bins, const int *s_m,
+                           int nd, int mmax, long long *lscr, double *dscr,
+                           int *iscr, double *dscr2, double *dscr3,
+                           long long *s_carry) {
     const int tid = threadIdx.x, os = b.oversample;
     const double period = b.periods[p], invP = 1.0 / period;
     const double bd = P.bd, inv_bd = 1.0 / bd;
@@ -1564,8 +1597,8 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
                         P.n < (1 << 28) &&
                         lds_bins - ncyc - mmax - 1 >= 64;
     const int S = ranged ? lds_bins - ncyc - mmax - 1 : lds_bins - mmax - 1;
-    int *rng = (int *)(binW + (lds_bins - ncyc));  // [2c], [2c+1]
-    int *erng = (int *)(binY + (lds_bins - ncyc));
+    int *rng = (int *)(bins + 2 * (lds_bins - ncyc));  // [2c], [2c+1]
+    int *erng = rng + 2 * ncyc;
     const long long *wn = w.nat_w + nbase;
     const long long *yn = w.nat_y + nbase;
     Score best;
@@ -1573,8 +1606,9 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
     const double *tp = w.ws_t + P.ws_off + tid;
     const long long *wp = w.ws_w + P.ws_off + tid;
     const long long *yp = w.ws_y + P.ws_off + tid;
-    double *Cw = (double *)binW;
-    double *Cy = (double *)binY;
+    // the prefix values, in place: C[2i] the weight (packed curves: the
+    // point count), C[2i + 1] the weight*flux sum
+    double *C = (double *)bins;
     if (tid == 0) {
         s_carry[0] = 0;
         s_carry[1] = 0;
@@ -1587,10 +1621,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
             cw = s_carry[0];
             cy = s_carry[1];
         }
-        for (int i = tid; i < len; i += NT) {
-            binW[i] = 0;
-            binY[i] = 0;
-        }
+        for (int i = tid; i < 2 * len; i += NT) bins[i] = 0;
         const bool extra = hi >= nb - os;  // padding copies land here
         if (ranged) {
             const int ntask = (extra ? 4 : 2) * ncyc;
@@ -1610,7 +1641,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
             // padding copies, bins 1..os of every cycle to nb-os..nb-1
 #define ATG_RANGE(F, PD, U, J0, E)                                           \
     big_fold_range<F, PD, U>(tn, wn, yn, J0, E, period, (double)c, bd,       \
-                             inv_bd, nb, os, s0, hi, binW, binY, P.wq_one)
+                             inv_bd, P.tol_x, nb, os, s0, hi, bins, P.wq_one)
 #define ATG_RANGES(PD, TAB)                                                  \
     for (int c = 0; c < ncyc; ++c) {                                         \
         const int j0 = TAB[2 * c], e = TAB[2 * c + 1];                       \
@@ -1624,6 +1655,15 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
     }
             ATG_RANGES(false, rng)
             if (extra) {
+                // the padding assignment: bins nb-os..nb-1 hold only the
+                // copies of bins 1..os
+                __syncthreads();
+                const int z0 = max(nb - os, s0), z1 = min(nb - 1, hi);
+                for (int i = z0 + tid; i <= z1; i += NT) {
+                    bins[2 * (i - s0)] = 0;
+                    bins[2 * (i - s0) + 1] = 0;
+                }
+                __syncthreads();
                 ATG_RANGES(true, erng)
             }
 #undef ATG_RANGES
@@ -1643,8 +1683,8 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
                                 nb);
                 int j1, j2;
                 pad_targets(r, nb, os, j1, j2);
-                seg_add(j1, s0, hi, wv, yv, cw, cy, binW, binY);
-                seg_add(j2, s0, hi, wv, yv, cw, cy, binW, binY);
+                seg_add(j1, s0, hi, wv, yv, cw, cy, bins);
+                seg_add(j2, s0, hi, wv, yv, cw, cy, bins);
             }
             __syncthreads();
             cw = block_reduce(cw, AddOp<long long>(), lscr);
@@ -1652,28 +1692,34 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
         }
         const int seg = (len + NT - 1) / NT;
         const int b0 = min(tid * seg, len), b1 = min(b0 + seg, len);
+        const longlong2 *B2 = (const longlong2 *)bins;
         long long sw = 0, sy = 0;
         for (int i = b0; i < b1; ++i) {
-            sw += binW[i];
-            sy += binY[i];
+            const longlong2 v = B2[i];
+            sw += v.x;
+            sy += v.y;
         }
         long long tw, ty;
         long long ow = block_excl_scan_dpp(sw, lscr, &tw) + cw;
         long long oy = block_excl_scan_dpp(sy, lscr, &ty) + cy;
         for (int i = b0; i < b1; ++i) {
-            ow += binW[i];
-            oy += binY[i];
+            const longlong2 v = B2[i];
+            ow += v.x;
+            oy += v.y;
             if (i == S - 1) {  // prefix of the next segment's first bin
                 s_carry[0] = ow;
                 s_carry[1] = oy;
             }
             // packed curves: the point count, as in finish_period (every
             // point adds the common weight wq_one)
-            Cw[i] = P.packed ? (double)(ow / P.wq_one) : (double)ow * fw;
-            Cy[i] = (double)oy * fy;
+            double2 cv;
+            cv.x = P.packed ? (double)(ow / P.wq_one) : (double)ow * fw;
+            cv.y = (double)oy * fy;
+            ((double2 *)C)[i] = cv;
         }
         __syncthreads();
         const int nend = min(s0 + S, nb);
+        const double2 *C2 = (const double2 *)C;
         // duration-major, as in finish_period (indices are not monotone
         // across segments, so the full tie rule stays)
         for (int k = 0; k < nd; ++k) {
@@ -1682,10 +1728,9 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
 #pragma unroll 4
             for (int n = s0 + tid; n <= last; n += NT) {
                 const int l = n - s0;
-                const double w0 = Cw[l], y0 = Cy[l];
-                const double we = Cw[l + m], ye = Cy[l + m];
-                const double win = P.packed ? (we - w0) * c1 : we - w0;
-                const double yin = ye - y0;
+                const double2 c0 = C2[l], ce = C2[l + m];
+                const double win = P.packed ? (ce.x - c0.x) * c1 : ce.x - c0.x;
+                const double yin = ce.y - c0.y;
                 const double wout = 1.0 - win;
                 const double num = fma(ytn, win, -yin);
                 const bool ok =
@@ -1762,17 +1807,6 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
         }
     }
     __syncthreads();
-}
-
-__device__ __forceinline__ void flush_run(long long *binW, long long *binY,
-                                          int at, long long aw,
-                                          long long ay) {
-    if (aw != 0 || ay != 0) {
-        __hip_atomic_fetch_add(&binW[at], aw, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(&binY[at], ay, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
 }
 
 // Four waves per SIMD (at most 128 VGPRs): the fold, the LDS atomics and
@@ -1988,8 +2022,7 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4)))
 void k_big(Batch b, Out o, Work w, int lds_bins) {
     extern __shared__ __align__(16) unsigned char smem[];
-    long long *binW = (long long *)smem;
-    long long *binY = binW + lds_bins;
+    long long *bins = (long long *)smem;  // lds_bins interleaved bins
     __shared__ int s_m[DMAX];
     __shared__ unsigned int s_j;
     __shared__ long long lscr[NW];
@@ -2030,7 +2063,7 @@ void k_big(Batch b, Out o, Work w, int lds_bins) {
         const int os = b.oversample;
         const int nbp = (int)ceil(b.periods[p] / P.bd) + os;
         if (nbp - os > os && lds_bins - mmax - 1 >= 1) {
-            big_period(b, o, w, P, p, nbp, lds_bins, binW, binY, s_m, nd,
+            big_period(b, o, w, P, p, nbp, lds_bins, bins, s_m, nd,
                        mmax, lscr, dscr, iscr, dscr2, dscr3, s_carry);
         } else if (tid == 0) {
             write_nan(o, p);
