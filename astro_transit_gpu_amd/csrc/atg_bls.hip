@@ -92,6 +92,7 @@ struct LcParams {
     int ybits;        // packed bin = count << ybits + weight*flux quantum
     long long wq_one; // the common quantized weight of a packed curve
     int uniform;      // every point has the same weight (wq_one)
+    double t_pad;     // t' of the zero-weight padding slots
 };
 
 struct Batch {
@@ -379,12 +380,19 @@ __device__ __forceinline__ T load_at(gchar *base, unsigned byte_off) {
 // oracle's and lies in [1, nb - os]; a flagged point (a superset of the
 // uncertain ones) is first added to its reciprocal bin, within [1, nb], and
 // then moved to the bin the oracle's divisions give.
-template <int C, bool EXACT, bool PACKED>
+// KIND: 0 unpacked bins (t', weight and weight*flux read per point),
+// 1 packed bins (t' and the packed increment), 2 unpacked bins of a curve
+// whose points share one weight wone (t' and weight*flux read; the npad
+// padding slots, of zero weight, at time t_pad, then also add wone to
+// t_pad's bin, which is taken back at the end).
+template <int C, bool EXACT, int KIND>
 __device__ __forceinline__ void bin_points(
     const double *tp, const long long *wp, const long long *yp, int L,
     const double *per, const double *inv, const int *nb, const int *base,
     double bd, double inv_bd, double tol_x, long long *binW,
-    long long *binY, const double *pg) {
+    long long *binY, const double *pg, long long wone, int npad,
+    double t_pad) {
+    constexpr bool PACKED = KIND == 1, UW = KIND == 2;
     // tp, wp, yp: the curve's (uniform) slot bases, in scalar registers;
     // lane tid loads slot tid + i * NT at a 32-bit byte offset from them
     // (base + offset addressing: no per-lane 64-bit address arithmetic; a
@@ -392,7 +400,7 @@ __device__ __forceinline__ void bin_points(
     gchar *bt = scalar_base(tp), *bw = scalar_base(wp), *by = scalar_base(yp);
     unsigned int lo = threadIdx.x * 8u;
     double tv_n = load_at<double>(bt, lo);
-    long long wv_n = load_at<long long>(bw, lo);
+    long long wv_n = UW ? wone : load_at<long long>(bw, lo);
     long long yv_n = PACKED ? 0 : load_at<long long>(by, lo);
     const double tol2 = 2.0 * tol_x;
     // per-period bin origins (bin 1 for the exact fold, bin 0 for the fast
@@ -412,7 +420,7 @@ __device__ __forceinline__ void bin_points(
         if (i + 1 < L) {
             lo += NT * 8;
             tv_n = load_at<double>(bt, lo);
-            wv_n = load_at<long long>(bw, lo);
+            if (!UW) wv_n = load_at<long long>(bw, lo);
             if (!PACKED) yv_n = load_at<long long>(by, lo);
         }
         int at[C];
@@ -500,6 +508,24 @@ __device__ __forceinline__ void bin_points(
                         }
                     }
                 }
+        }
+    }
+    if (UW && npad > 0 && threadIdx.x == 0) {
+        // the padding slots' weight, added above to t_pad's bin (the bin
+        // the fold of this mode gives it), taken back (integer adds commute)
+#pragma unroll
+        for (int s = 0; s < C; ++s) {
+            int r;
+            if (EXACT) {
+                r = min(max(fold_exact(t_pad, pg[s], bd), 0), nb[s]);
+            } else {
+                const double ph =
+                    fma(-per[s], floor(t_pad * inv[s]), t_pad);
+                r = min(max((int)(ph * inv_bd) + 1, 0), nb[s]);
+            }
+            __hip_atomic_fetch_add(&binW[base[s] + r], -wone * npad,
+                                   __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
 }
@@ -899,6 +925,7 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     P.L = L0 + L1;
     long long wq_sum = 0, yq_sum = 0;
     const double t_pad = tmax - tmin;
+    P.t_pad = t_pad;
     const long long yq_pad = 0;
     // padding slots of this lane: zero weight, an in-range time
     for (int i = 0; i < L0; ++i)
@@ -1968,7 +1995,8 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
 #define ATG_BIN(C, E, K)                                                    \
     case C:                                                                 \
         bin_points<C, E, K>(tp, wp, yp, L, per, inv, nb, base, bd, inv_bd,  \
-                            P.tol_x, binW, binY, b.periods + p);            \
+                            P.tol_x, binW, binY, b.periods + p, P.wq_one,   \
+                            L * NT - P.n, P.t_pad);                         \
         break;
 #define ATG_BIN8(E, K)                                                      \
     switch (cnt) {                                                          \
@@ -1976,8 +2004,9 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
         ATG_BIN(5, E, K) ATG_BIN(6, E, K) ATG_BIN(7, E, K) ATG_BIN(8, E, K) \
         default: break;                                                     \
     }
-                if (packed) ATG_BIN8(EXACT, true)
-                else ATG_BIN8(EXACT, false)
+                if (packed) ATG_BIN8(EXACT, 1)
+                else if (P.uniform) ATG_BIN8(EXACT, 2)
+                else ATG_BIN8(EXACT, 0)
 #undef ATG_BIN8
 #undef ATG_BIN
             }
