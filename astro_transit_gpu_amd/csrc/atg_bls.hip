@@ -1583,6 +1583,36 @@ __device__ __forceinline__ void big_fold_range(
     if (Be < e) fold_block(Be, true);
 }
 
+// Box scan of one segment of a phase-tiled period: starts s0..nend-1 of
+// every duration over the segment's prefix values C2[l] = (weight, flux)
+// of bin s0 + l (packed curves: the point count as weight, times c1), the
+// thread taking starts s0 + tid, s0 + tid + NT, ...  sb receives the
+// thread's best window (the first of equal scores: its indices increase).
+template <bool PACKED, bool LIKE>
+__device__ __forceinline__ void big_box_scan(Score &sb, const double2 *C2,
+                                             int s0, int nend, int nb,
+                                             const int *s_m, int nd,
+                                             double c1, double ytn,
+                                             double epsn) {
+    const int nval = nb + 1;
+    for (int k = 0; k < nd; ++k) {
+        const int m = s_m[k];
+        const int last = min(nend - 1, nb - m);
+#pragma unroll 4
+        for (int n = s0 + (int)threadIdx.x; n <= last; n += NT) {
+            const int l = n - s0;
+            const double2 c0 = C2[l], ce = C2[l + m];
+            const double win = PACKED ? (ce.x - c0.x) * c1 : ce.x - c0.x;
+            const double yin = ce.y - c0.y;
+            const double wout = 1.0 - win;
+            const double num = fma(ytn, win, -yin);
+            sb.offer_later(!(fmin(win, wout) < epsn) & !(num < 0.0),
+                           num * num, LIKE ? win * wout * wout : win * wout,
+                           k * nval + n);
+        }
+    }
+}
+
 // One period whose n_bins + 1 exceeds the LDS histogram: the start
 // positions are processed in segments of S start positions; segment k keeps
 // bins [s0, s0 + S + mmax] in LDS (interleaved weight and weight*flux
@@ -1747,27 +1777,27 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
         __syncthreads();
         const int nend = min(s0 + S, nb);
         const double2 *C2 = (const double2 *)C;
-        // duration-major, as in finish_period (indices are not monotone
-        // across segments, so the full tie rule stays)
-        for (int k = 0; k < nd; ++k) {
-            const int m = s_m[k];
-            const int last = min(nend - 1, nb - m);
-#pragma unroll 4
-            for (int n = s0 + tid; n <= last; n += NT) {
-                const int l = n - s0;
+        // duration-major, as in finish_period: a thread meets the segment's
+        // windows in increasing index, so the segment's best needs no tie
+        // rule; it joins the period's best with the full one (indices are
+        // not monotone across segments)
+        Score sb;
+        if (P.packed) {
+            if (like) big_box_scan<true, true>(sb, C2, s0, nend, nb, s_m, nd, c1, ytn, epsn);
+            else big_box_scan<true, false>(sb, C2, s0, nend, nb, s_m, nd, c1, ytn, epsn);
+        } else {
+            if (like) big_box_scan<false, true>(sb, C2, s0, nend, nb, s_m, nd, c1, ytn, epsn);
+            else big_box_scan<false, false>(sb, C2, s0, nend, nb, s_m, nd, c1, ytn, epsn);
+        }
+        if (sb.idx != INT_MAX) {
+            const int before = best.idx;
+            best.offer(sb.q, sb.den, sb.idx);
+            if (best.idx != before) {  // the window's sums, read back
+                const int k = sb.idx / nval, n = sb.idx - k * nval;
+                const int l = n - s0, m = s_m[k];
                 const double2 c0 = C2[l], ce = C2[l + m];
-                const double win = P.packed ? (ce.x - c0.x) * c1 : ce.x - c0.x;
-                const double yin = ce.y - c0.y;
-                const double wout = 1.0 - win;
-                const double num = fma(ytn, win, -yin);
-                const bool ok =
-                    !(win < epsn) & !(wout < epsn) & !(num < 0.0);
-                if (best.offer_if(ok, num * num,
-                                  like ? win * wout * wout : win * wout,
-                                  k * nval + n)) {
-                    bwin = win;
-                    byin = yin;
-                }
+                bwin = P.packed ? (ce.x - c0.x) * c1 : ce.x - c0.x;
+                byin = ce.y - c0.y;
             }
         }
         __syncthreads();
