@@ -403,13 +403,13 @@ __device__ __forceinline__ void bin_points(
     long long wv_n = UW ? wone : load_at<long long>(bw, lo);
     long long yv_n = PACKED ? 0 : load_at<long long>(by, lo);
     const double tol2 = 2.0 * tol_x;
-    // per-period bin origins (bin 1 for the exact fold, bin 0 for the fast
-    // one), so that a bin's LDS address is one shift-and-add of its index
+    // per-period origins at bin 1, so that bin at + 1's LDS address is one
+    // shift-and-add of at
     long long *ow[C], *oy[C];
 #pragma unroll
     for (int s = 0; s < C; ++s) {
-        ow[s] = binW + base[s] + (EXACT ? 1 : 0);
-        oy[s] = binY + base[s] + (EXACT ? 1 : 0);
+        ow[s] = binW + base[s] + 1;
+        oy[s] = binY + base[s] + 1;
     }
     // two points per iteration: the prefetched registers alternate instead
     // of being copied
@@ -456,9 +456,15 @@ __device__ __forceinline__ void bin_points(
                 at[s] = (int)x;  // bin at[s] + 1
             } else {
                 // fast mode: FMA phase and rounded reciprocals only; a point
-                // within ~1e-15 of a bin edge may land in the neighbour bin
+                // within ~1e-15 of a bin edge may land in the neighbour bin.
+                // No clamp is needed: t' >= 0, and t' * inv is within
+                // t'/P * 2^-52 of t'/P, so the cycle is off by at most one
+                // and then the phase lies within t' * 2^-52 (far below bd)
+                // outside [0, P): the truncated quotient is in
+                // [0, ceil(P/bd)], at most nb - 1 (oversample >= 1), and
+                // the bin (at + 1) is the clamped one of fold_bin_in
                 const double ph = fma(-per[s], floor(tv * inv[s]), tv);
-                at[s] = min(max((int)(ph * inv_bd) + 1, 0), nb[s]);
+                at[s] = (int)(ph * inv_bd);
             }
         }
 #pragma unroll
