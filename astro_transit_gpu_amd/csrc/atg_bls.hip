@@ -66,6 +66,9 @@ constexpr int EHIST = 1075 + 1024;  // k_prep's exponent histogram
 // entries past a period's prefix arrays that k_main's box scan reads (the
 // starts of a duration's last stride beyond its final window)
 constexpr int PADN = NT - 1;
+// bins of k_main's histogram in counted form (12 B each: an int64 flux sum
+// and an int32 point count) in the LDS of lds_bins 16-byte bins
+constexpr int counted_cap(int lds_bins) { return lds_bins * 4 / 3; }
 
 struct LcParams {
     double t_ref;     // min(t)
@@ -380,18 +383,19 @@ __device__ __forceinline__ T load_at(gchar *base, unsigned byte_off) {
 // oracle's and lies in [1, nb - os]; a flagged point (a superset of the
 // uncertain ones) is first added to its reciprocal bin, within [1, nb], and
 // then moved to the bin the oracle's divisions give.
-// KIND: 0 unpacked bins (t', weight and weight*flux read per point),
-// 1 packed bins (t' and the packed increment), 2 unpacked bins of a curve
-// whose points share one weight wone (t' and weight*flux read; the npad
-// padding slots, of zero weight, at time t_pad, then also add wone to
-// t_pad's bin, which is taken back at the end).
+// KIND: 0 unpacked bins (t', weight and weight*flux read per point; int64
+// weight sums in binW, int64 flux sums in binY), 1 packed bins (t' and the
+// packed increment, into binW), 2 counted bins of a curve whose points
+// share one weight (t' and weight*flux read; int32 point counts in binW,
+// read as int, int64 flux sums in binY: 12 bytes a bin).  KIND 2 counts
+// the npad padding slots (zero weight, at time t_pad) too, and takes them
+// back from t_pad's bin at the end.
 template <int C, bool EXACT, int KIND>
 __device__ __forceinline__ void bin_points(
     const double *tp, const long long *wp, const long long *yp, int L,
     const double *per, const double *inv, const int *nb, const int *base,
     double bd, double inv_bd, double tol_x, long long *binW,
-    long long *binY, const double *pg, long long wone, int npad,
-    double t_pad) {
+    long long *binY, const double *pg, int npad, double t_pad) {
     constexpr bool PACKED = KIND == 1, UW = KIND == 2;
     // tp, wp, yp: the curve's (uniform) slot bases, in scalar registers;
     // lane tid loads slot tid + i * NT at a 32-bit byte offset from them
@@ -400,16 +404,18 @@ __device__ __forceinline__ void bin_points(
     gchar *bt = scalar_base(tp), *bw = scalar_base(wp), *by = scalar_base(yp);
     unsigned int lo = threadIdx.x * 8u;
     double tv_n = load_at<double>(bt, lo);
-    long long wv_n = UW ? wone : load_at<long long>(bw, lo);
+    long long wv_n = UW ? 1 : load_at<long long>(bw, lo);
     long long yv_n = PACKED ? 0 : load_at<long long>(by, lo);
     const double tol2 = 2.0 * tol_x;
     // per-period origins at bin 1, so that bin at + 1's LDS address is one
     // shift-and-add of at
     long long *ow[C], *oy[C];
+    int *oc[C];
 #pragma unroll
     for (int s = 0; s < C; ++s) {
         ow[s] = binW + base[s] + 1;
         oy[s] = binY + base[s] + 1;
+        oc[s] = (int *)binW + base[s] + 1;
     }
     // two points per iteration: the prefetched registers alternate instead
     // of being copied
@@ -469,8 +475,12 @@ __device__ __forceinline__ void bin_points(
         }
 #pragma unroll
         for (int s = 0; s < C; ++s) {
-            __hip_atomic_fetch_add(ow[s] + at[s], wv, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (UW)
+                __hip_atomic_fetch_add(oc[s] + at[s], 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            else
+                __hip_atomic_fetch_add(ow[s] + at[s], wv, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
             if (!PACKED)
                 __hip_atomic_fetch_add(oy[s] + at[s], yv, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -500,10 +510,21 @@ __device__ __forceinline__ void bin_points(
                     const int e =
                         base[s] + min(max((int)trunc(ph / bd) + 1, 0), nb[s]);
                     if (e != a) {
-                        __hip_atomic_fetch_add(&binW[a], -wv, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                        __hip_atomic_fetch_add(&binW[e], wv, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (UW) {
+                            __hip_atomic_fetch_add(&((int *)binW)[a], -1,
+                                                   __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                            __hip_atomic_fetch_add(&((int *)binW)[e], 1,
+                                                   __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                        } else {
+                            __hip_atomic_fetch_add(&binW[a], -wv,
+                                                   __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                            __hip_atomic_fetch_add(&binW[e], wv,
+                                                   __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
                         if (!PACKED) {
                             __hip_atomic_fetch_add(
                                 &binY[a], -yv, __ATOMIC_RELAXED,
@@ -517,8 +538,8 @@ __device__ __forceinline__ void bin_points(
         }
     }
     if (UW && npad > 0 && threadIdx.x == 0) {
-        // the padding slots' weight, added above to t_pad's bin (the bin
-        // the fold of this mode gives it), taken back (integer adds commute)
+        // the padding slots, counted above in t_pad's bin (the bin the fold
+        // of this mode gives it), taken back (integer adds commute)
 #pragma unroll
         for (int s = 0; s < C; ++s) {
             int r;
@@ -529,7 +550,7 @@ __device__ __forceinline__ void bin_points(
                     fma(-per[s], floor(t_pad * inv[s]), t_pad);
                 r = min(max((int)(ph * inv_bd) + 1, 0), nb[s]);
             }
-            __hip_atomic_fetch_add(&binW[base[s] + r], -wone * npad,
+            __hip_atomic_fetch_add(&((int *)binW)[base[s] + r], -npad,
                                    __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
         }
@@ -1003,9 +1024,11 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     // overhang of its box scan)
     const double ov =
         fmax(0.0, PADN + fmin(round(dmax / P.bd), 32767.0) - nbmax);
-    const bool big = P.packed ? nbmax + 1 + fmax(floor((nbmax + 1 + ov + 1) / 2),
-                                                 ov) > 2.0 * lds_bins
-                              : nbmax + 1 + ov > (double)lds_bins;
+    const bool big =
+        P.packed ? nbmax + 1 + fmax(floor((nbmax + 1 + ov + 1) / 2), ov) >
+                       2.0 * lds_bins
+                 : nbmax + 1 + ov > (double)(P.uniform ? counted_cap(lds_bins)
+                                                       : lds_bins);
     const bool big_exact = b.mode == ATG_MODE_PARITY &&
                            nbmax + 1 > (double)min(lds_bins, EXACT_BINS);
     if (P.sorted && (big || big_exact)) {
@@ -1329,7 +1352,9 @@ __device__ __forceinline__ int block_argmax(const Score &best, double *dscr,
 // occupies bins [base, base + nb] of binW / binY; writes period p.  A
 // packed histogram (P.packed) lives in binW alone (binY is unused): its
 // scan writes the FP64 flux prefix in place and the int32 point counts into
-// the (nb + 2) / 2 slots above it.
+// the (nb + 2) / 2 slots above it.  Counted bins (cnt non-null: a
+// uniform-weight curve with unpacked bins) keep the int64 flux sums in binW
+// and the int32 point counts in cnt; both prefixes are written in place.
 // Entries past a period's last prefix value that its box scan reads: the
 // lanes past the final window of a duration with fewer than 64 starts reach
 // index 63 + m (m the duration's bins), so max(0, PADN + m_max - nb).
@@ -1337,25 +1362,29 @@ __device__ __forceinline__ int scan_overhang(int nb, int mmax) {
     return max(0, PADN + mmax - nb);
 }
 
+
 template <bool LIKE>
 __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
                               long long p, double period, int nb, int base,
-                              long long *binW, long long *binY,
+                              long long *binW, long long *binY, int *cnt,
                               const short *s_m, int nd, int mmax,
                               long long *lscr, double *dscr, int *iscr) {
     const int tid = threadIdx.x;
     const int os = b.oversample;
     const bool packed = P.packed;
+    const bool counted = cnt != nullptr;
     // padding: bins[nb-os+j-1] = bins[j], j = 1..os (an assignment)
     if (nb - os > os) {
         for (int j = 1 + tid; j <= os; j += NT) {
             binW[base + nb - os + j - 1] = binW[base + j];
-            if (!packed) binY[base + nb - os + j - 1] = binY[base + j];
+            if (counted) cnt[base + nb - os + j - 1] = cnt[base + j];
+            else if (!packed) binY[base + nb - os + j - 1] = binY[base + j];
         }
     } else if (tid == 0) {  // overlapping ranges: the oracle's order
         for (int j = 1, d = nb - os; j <= os; ++j, ++d) {
             binW[base + d] = binW[base + j];
-            if (!packed) binY[base + d] = binY[base + j];
+            if (counted) cnt[base + d] = cnt[base + j];
+            else if (!packed) binY[base + d] = binY[base + j];
         }
     }
     __syncthreads();
@@ -1366,8 +1395,13 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
     const int b0 = min(tid * seg, nval), b1 = min(b0 + seg, nval);
     long long sw = 0, sy = 0;
     for (int i = b0; i < b1; ++i) {
-        sw += binW[base + i];
-        if (!packed) sy += binY[base + i];
+        if (counted) {
+            sw += cnt[base + i];
+            sy += binW[base + i];
+        } else {
+            sw += binW[base + i];
+            if (!packed) sy += binY[base + i];
+        }
     }
     long long totw, toty;
     long long ow = block_excl_scan_dpp(sw, lscr, &totw);
@@ -1380,13 +1414,21 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
     const double fw = P.w_unit / P.W_tot, fy = P.y_unit / P.W_tot;
     const double c1 = (double)P.wq_one * fw;
     double *Cw = (double *)binW + base;
-    double *Cy = packed ? (double *)binW + base : (double *)binY + base;
-    int *Cn = (int *)(binW + base + nval);
+    double *Cy = (packed || counted) ? (double *)binW + base
+                                     : (double *)binY + base;
+    int *Cn = counted ? cnt + base : (int *)(binW + base + nval);
     // packed prefix = (points << ybits) + flux sum, exactly; ybits >= 40,
     // so the low word is the flux field's and the high word holds the
     // count above its top hb = ybits - 32 (sign-extended) flux bits
     const int hb = P.ybits - 32;
     for (int i = b0; i < b1; ++i) {
+        if (counted) {
+            ow += cnt[base + i];
+            oy += binW[base + i];
+            Cn[i] = (int)ow;
+            Cy[i] = (double)oy * fy;
+            continue;
+        }
         ow += binW[base + i];
         if (packed) {
             const int hi = (int)((unsigned long long)ow >> 32);
@@ -1407,7 +1449,9 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
     // but never decide a window)
     const int over = scan_overhang(nb, mmax);
     for (int t = tid; t < over; t += NT) {
-        if (packed) {
+        if (counted) {
+            Cn[nval + t] = (int)totw + (1 << 30);
+        } else if (packed) {
             const int hi = (int)((unsigned long long)totw >> 32);
             const int yhi = (int)((unsigned)hi << (32 - hb)) >> (32 - hb);
             Cn[nval + t] = (int)((unsigned)(hi - yhi) >> hb) + (1 << 30);
@@ -1421,7 +1465,7 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
     const double Wt = P.W_tot, Yt = P.Y_tot;
     const double ytn = Yt / Wt, epsn = DBL_EPSILON / Wt;
     Score best;
-    if (packed)
+    if (packed || counted)
         box_scan_pad<true, LIKE>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn, epsn);
     else
         box_scan_pad<false, LIKE>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn,
@@ -1431,7 +1475,7 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
     if (bi == INT_MAX) {  // no window of positive depth: the oracle's choice
                           // among those of depth 0, if any
         Score zero;
-        if (packed)
+        if (packed || counted)
             box_scan<true, LIKE>(zero, Cw, Cn, Cy, c1, s_m, nd, nb, ytn, epsn);
         else
             box_scan<false, LIKE>(zero, Cw, Cn, Cy, c1, s_m, nd, nb, ytn,
@@ -1452,9 +1496,9 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
             // from them in parallel over periods (the duration slot carries
             // -(index + 1) until then)
             const int k = bi / nval, s = bi - k * nval, m = s_m[k];
-            o.depth[p] =
-                (packed ? (double)(Cn[s + m] - Cn[s]) * c1 : Cw[s + m] - Cw[s]) *
-                Wt;
+            o.depth[p] = ((packed || counted) ? (double)(Cn[s + m] - Cn[s]) * c1
+                                              : Cw[s + m] - Cw[s]) *
+                         Wt;
             o.depth_err[p] = (Cy[s + m] - Cy[s]) * Wt;
             o.duration[p] = -(double)bi - 1.0;
         }
@@ -1591,7 +1635,7 @@ __device__ __forceinline__ void big_fold_range(
 
 // Box scan of one segment of a phase-tiled period: starts s0..nend-1 of
 // every duration over the segment's prefix values C2[l] = (weight, flux)
-// of bin s0 + l (packed curves: the point count as weight, times c1), the
+// of bin s0 + l (uniform weights: the point count as weight, times c1), the
 // thread taking starts s0 + tid, s0 + tid + NT, ...  sb receives the
 // thread's best window (the first of equal scores: its indices increase).
 template <bool PACKED, bool LIKE>
@@ -1669,7 +1713,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
     const double *tp = w.ws_t + P.ws_off + tid;
     const long long *wp = w.ws_w + P.ws_off + tid;
     const long long *yp = w.ws_y + P.ws_off + tid;
-    // the prefix values, in place: C[2i] the weight (packed curves: the
+    // the prefix values, in place: C[2i] the weight (uniform weights: the
     // point count), C[2i + 1] the weight*flux sum
     double *C = (double *)bins;
     if (tid == 0) {
@@ -1773,10 +1817,10 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
                 s_carry[0] = ow;
                 s_carry[1] = oy;
             }
-            // packed curves: the point count, as in finish_period (every
+            // uniform weights: the point count, as in finish_period (every
             // point adds the common weight wq_one)
             double2 cv;
-            cv.x = P.packed ? (double)(ow / P.wq_one) : (double)ow * fw;
+            cv.x = P.uniform ? (double)(ow / P.wq_one) : (double)ow * fw;
             cv.y = (double)oy * fy;
             ((double2 *)C)[i] = cv;
         }
@@ -1788,7 +1832,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
         // rule; it joins the period's best with the full one (indices are
         // not monotone across segments)
         Score sb;
-        if (P.packed) {
+        if (P.uniform) {
             if (like) big_box_scan<true, true>(sb, C2, s0, nend, nb, s_m, nd, c1, ytn, epsn);
             else big_box_scan<true, false>(sb, C2, s0, nend, nb, s_m, nd, c1, ytn, epsn);
         } else {
@@ -1802,7 +1846,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
                 const int k = sb.idx / nval, n = sb.idx - k * nval;
                 const int l = n - s0, m = s_m[k];
                 const double2 c0 = C2[l], ce = C2[l + m];
-                bwin = P.packed ? (ce.x - c0.x) * c1 : ce.x - c0.x;
+                bwin = P.uniform ? (ce.x - c0.x) * c1 : ce.x - c0.x;
                 byin = ce.y - c0.y;
             }
         }
@@ -1962,8 +2006,14 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
             // packed histogram takes 8 B per bin, so both halves of the LDS
             // hold bins; each packed period also needs (nb + 2) / 2 free
             // slots above it for its point counts (finished last to first).
+            // Counted bins (uniform weights, unpacked) take 12 B: int64
+            // flux sums from the start of the LDS, int32 counts after
+            // lds_bins * 4 / 3 of them.
             const bool packed = P.packed;
-            const int cap = packed ? 2 * lds_bins : lds_bins;
+            const bool counted = !packed && P.uniform;
+            const int cap2 = counted_cap(lds_bins);
+            const int cap = packed ? 2 * lds_bins : (counted ? cap2 : lds_bins);
+            int *cntb = (int *)(binW + cap2);
             double per[SUBMAX], inv[SUBMAX];
             int nb[SUBMAX], base[SUBMAX];
             int cnt = 0, used = 0;
@@ -2014,7 +2064,8 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
             }
             for (int i = tid; i < used; i += NT) {
                 binW[i] = 0;
-                if (!packed) binY[i] = 0;
+                if (counted) cntb[i] = 0;
+                else if (!packed) binY[i] = 0;
             }
             __syncthreads();
 
@@ -2028,11 +2079,14 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
                 const long long *wp = w.ws_w + P.ws_off;
                 const long long *yp = w.ws_y + P.ws_off;
                 const int L = P.L;
+                // counted bins: the counts take the weight channel
+                long long *hW = counted ? (long long *)cntb : binW;
+                long long *hY = counted ? binW : binY;
 #define ATG_BIN(C, E, K)                                                    \
     case C:                                                                 \
         bin_points<C, E, K>(tp, wp, yp, L, per, inv, nb, base, bd, inv_bd,  \
-                            P.tol_x, binW, binY, b.periods + p, P.wq_one,   \
-                            L * NT - P.n, P.t_pad);                         \
+                            P.tol_x, hW, hY, b.periods + p, L * NT - P.n,   \
+                            P.t_pad);                                       \
         break;
 #define ATG_BIN8(E, K)                                                      \
     switch (cnt) {                                                          \
@@ -2055,7 +2109,8 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
                     const int nbs = (int)ceil(ps / bd) + os;
                     const int bse = end - (nbs + 1);
                     finish_period<LIKE>(b, o, P, p + s, ps, nbs, bse, binW, binW,
-                                  s_m, nd, mmax, lscr, dscr, iscr);
+                                        nullptr, s_m, nd, mmax, lscr, dscr,
+                                        iscr);
                     end = bse;
                 }
             } else {
@@ -2067,7 +2122,8 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
                     const int nbs = (int)ceil(ps / bd) + os;
                     const int bse = end - (nbs + 1);
                     finish_period<LIKE>(b, o, P, p + s, ps, nbs, bse, binW, binY,
-                                  s_m, nd, mmax, lscr, dscr, iscr);
+                                        counted ? cntb : nullptr, s_m, nd, mmax,
+                                        lscr, dscr, iscr);
                     end = bse;
                 }
             }
