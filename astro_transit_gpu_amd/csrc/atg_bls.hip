@@ -1703,9 +1703,28 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
     const bool ranged = P.natural && ncyc <= (1 << 20) &&
                         P.n < (1 << 28) &&
                         lds_bins - ncyc - mmax - 1 >= 64;
-    const int S = ranged ? lds_bins - ncyc - mmax - 1 : lds_bins - mmax - 1;
-    int *rng = (int *)(bins + 2 * (lds_bins - ncyc));  // [2c], [2c+1]
-    int *erng = rng + 2 * ncyc;
+    int S = ranged ? lds_bins - ncyc - mmax - 1 : lds_bins - mmax - 1;
+    // The range table takes R 16-byte bins at the tail of the tile.  Where
+    // it costs at most an eighth of the tile, the ranges of every segment
+    // are searched at once, before the first segment (one round of binary
+    // searches per period instead of one per segment): [k][c][2] for
+    // segment k and cycle c, then the padding ranges [c][2]; otherwise
+    // each segment searches its own ranges into [c][2].
+    int R = ncyc, nseg = 0;
+    bool batched = false;
+    if (ranged) {
+        const int nseg0 = (nb + S - 1) / S;
+        const int Rb = (2 * ncyc * (nseg0 + 2) + 3) / 4;
+        const int Sb = lds_bins - Rb - mmax - 1;
+        if (Sb >= 64 && Rb <= lds_bins / 8 && (nb + Sb - 1) / Sb <= nseg0 + 1) {
+            batched = true;
+            R = Rb;
+            S = Sb;
+            nseg = (nb + Sb - 1) / Sb;
+        }
+    }
+    int *tab = (int *)(bins + 2 * (lds_bins - R));
+    int *erng = tab + 2 * ncyc * (batched ? nseg : 1);
     const long long *wn = w.nat_w + nbase;
     const long long *yn = w.nat_y + nbase;
     Score best;
@@ -1720,9 +1739,21 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
         s_carry[0] = 0;
         s_carry[1] = 0;
     }
+    if (batched) {
+        const int ntask = 2 * ncyc * (nseg + 1);
+        for (int e = tid; e < ntask; e += NT) {
+            const int k = e / (2 * ncyc), c = (e - k * 2 * ncyc) >> 1;
+            const int sb = k < nseg ? ((e & 1) ? min(k * S + S + mmax, nb) + 1
+                                               : k * S)
+                                    : ((e & 1) ? os + 1 : 1);
+            tab[e] = lower_key(tn, P.n, period, invP, bd, inv_bd, nb,
+                               (double)c, sb, fastm);
+        }
+    }
     __syncthreads();
-    for (int s0 = 0; s0 < nb; s0 += S) {
+    for (int s0 = 0, seg_k = 0; s0 < nb; s0 += S, ++seg_k) {
         const int hi = min(s0 + S + mmax, nb), len = hi - s0 + 1;
+        int *rng = tab + (batched ? 2 * ncyc * seg_k : 0);
         long long cw = 0, cy = 0;
         if (ranged) {
             cw = s_carry[0];
@@ -1730,7 +1761,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
         }
         for (int i = tid; i < 2 * len; i += NT) bins[i] = 0;
         const bool extra = hi >= nb - os;  // padding copies land here
-        if (ranged) {
+        if (ranged && !batched) {
             const int ntask = (extra ? 4 : 2) * ncyc;
             for (int e = tid; e < ntask; e += NT) {
                 const int c = (e >> 1) % ncyc;
