@@ -164,6 +164,14 @@ __device__ __forceinline__ double readlane_d(double v, int j) {
                                             lo));
 }
 
+// (double)v, exactly, for |v| < 2^51: the bits of 1.5 * 2^52 plus v are
+// the FP64 value 1.5 * 2^52 + v (ulp 1 there), and subtracting 1.5 * 2^52
+// leaves v (one integer add and one FP64 subtraction instead of the
+// two-word conversion).
+__device__ __forceinline__ double small_i64_to_f64(long long v) {
+    return __longlong_as_double(v + 0x4338000000000000ll) - 0x1.8p52;
+}
+
 // Order-preserving unsigned key of an FP64 value (non-NaN).
 __device__ __forceinline__ unsigned long long okey(double v) {
     unsigned long long u = (unsigned long long)__double_as_longlong(v);
@@ -874,8 +882,10 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
                 (b.mode == ATG_MODE_FAST || n <= 65536)) ? 1 : 0;
     P.ybits = 63 - cbits;
     // scale so that the sum of |quantized| values stays below 2^60 (packed:
-    // 2^(ybits-3)); the padding copies at most double a prefix sum.
-    const int ytop = P.packed ? P.ybits - 4 : 59;
+    // 2^(ybits-3), and 2^49 at most, so that every prefix, which the
+    // padding copies at most double, stays below 2^51 for the exact
+    // conversion of the scan, small_i64_to_f64).
+    const int ytop = P.packed ? min(P.ybits - 4, 48) : 59;
     int sw = 59 - ilogb(sw_sum);
     int sy = (sy_sum > 0.0 && isfinite(sy_sum)) ? ytop - ilogb(sy_sum) : 0;
     sw = max(-1000, min(1000, sw));
@@ -1185,26 +1195,31 @@ __device__ __forceinline__ double fold_mod(double x, double P) {
 
 // Sliding-box scan of one period's prefix arrays (Cw or the point counts
 // Cn, and Cy; index 0..nb).  Duration-major: a thread meets its windows in
-// increasing index (k * nval + s), so a later equal score never replaces
-// the held one.  Every lane runs the same number of iterations per duration
-// (the last one predicated off past the final start, its loads clamped to
-// it), so that a wave never runs a lane's remainder iterations on their own;
-// the window tests are a predicate rather than branches, so that the
-// unrolled iterations issue all their LDS loads before the first wait.
+// increasing index (k << 16) + s (k < 64 durations, s < 2^15 starts: the
+// order of (k, s), decoded with a shift), so a later equal score never
+// replaces the held one.  Every lane runs the same number of iterations per
+// duration (the last one predicated off past the final start, its loads
+// clamped to it), so that a wave never runs a lane's remainder iterations on
+// their own.
 // One window of the scan: start s, m bins, (duration, start) index idx;
-// counts only if in.
+// counts only if in.  ntot > 0 (counted weights with one point's weight at
+// least epsn, as box_scan_pad's CFORM): the weight tests are 0 < n < ntot
+// on the window's point count n, exactly.
 template <bool PACKED, bool LIKE>
 __device__ __forceinline__ void box_window(Score &best, const double *Cw,
                                            const int *Cn, const double *Cy,
                                            double c1, double ytn, double epsn,
-                                           int s, int m, int idx, bool in) {
-    const double win =
-        PACKED ? (double)(Cn[s + m] - Cn[s]) * c1 : Cw[s + m] - Cw[s];
+                                           int ntot, int s, int m, int idx,
+                                           bool in) {
+    const int ni = PACKED ? Cn[s + m] - Cn[s] : 0;
+    const double win = PACKED ? (double)ni * c1 : Cw[s + m] - Cw[s];
     const double yin = Cy[s + m] - Cy[s];
     const double wout = 1.0 - win;
     // depth * W_in * W_out / W_tot^2 = y_tot*w_in - y_in
     const double num = fma(ytn, win, -yin);
-    const bool ok = in & !(win < epsn) & !(wout < epsn) & !(num < 0.0);
+    const bool wok = ntot > 0 ? (ni > 0) & (ni < ntot)
+                              : !(win < epsn) & !(wout < epsn);
+    const bool ok = in & wok & !(num < 0.0);
     best.offer_later(ok, num * num, LIKE ? win * wout * wout : win * wout,
                      idx);
 }
@@ -1213,8 +1228,8 @@ template <bool PACKED, bool LIKE>
 __device__ __forceinline__ void box_scan(Score &best, const double *Cw,
                                          const int *Cn, const double *Cy,
                                          double c1, const short *s_m, int nd,
-                                         int nb, double ytn, double epsn) {
-    const int nval = nb + 1;
+                                         int nb, double ytn, double epsn,
+                                         int ntot) {
     for (int k = 0; k < nd; ++k) {
         const int m = s_m[k];
         const int send = min(nb - 1, nb - m);
@@ -1222,17 +1237,17 @@ __device__ __forceinline__ void box_scan(Score &best, const double *Cw,
         // the lanes past the final start are predicated off (their loads
         // clamped to it)
         const int full = (send + 1) / NT;
-        const int kbase = k * nval;
+        const int kbase = k << 16;
         const int s0 = (int)threadIdx.x;
 #pragma unroll 4
         for (int i = 0; i < full; ++i) {
             const int s = s0 + i * NT;
-            box_window<PACKED, LIKE>(best, Cw, Cn, Cy, c1, ytn, epsn, s, m,
-                                     kbase + s, true);
+            box_window<PACKED, LIKE>(best, Cw, Cn, Cy, c1, ytn, epsn, ntot, s,
+                                     m, kbase + s, true);
         }
         if (full * NT <= send) {
             const int s = s0 + full * NT;
-            box_window<PACKED, LIKE>(best, Cw, Cn, Cy, c1, ytn, epsn,
+            box_window<PACKED, LIKE>(best, Cw, Cn, Cy, c1, ytn, epsn, ntot,
                                      min(s, send), m, kbase + s, s <= send);
         }
     }
@@ -1252,16 +1267,26 @@ __device__ __forceinline__ void box_scan(Score &best, const double *Cw,
 // initial best (q = 0): only windows of positive depth are taken here, and
 // a period without one is rescanned with box_scan (the depth-0 windows the
 // oracle accepts).
-template <bool PACKED, bool LIKE>
+template <bool PACKED, bool LIKE, bool CFORM>
 __device__ __forceinline__ void box_scan_pad(Score &best, const double *Cw,
                                              const int *Cn, const double *Cy,
                                              double c1, const short *s_m,
                                              int nd, int nb, double ytn,
-                                             double epsn) {
-    const int nval = nb + 1;
+                                             double epsn, int ntot) {
     const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int lane = (int)threadIdx.x & 63;
     best.q = 0.0;
+    // CFORM (counted weights, packed or counted bins, with one point's
+    // weight c1 at least epsn): a window of n of the curve's ntot points
+    // weighs n*c1 and leaves (ntot - n)*c1 outside, so the weight tests
+    // fail exactly for n = 0 (an empty window, whose flux sum is 0 and
+    // whose score, 0, never beats the initial best) and for n >= ntot (the
+    // window holds every point, and padding copies), one integer compare.
+    // c1 then moves into the constants: wout = 1 - c1*n and
+    // num = ytn*c1*n - yin, and the score q/den is the window's divided by
+    // c1 (one factor for the whole period, so its windows keep their
+    // order).
+    const double ytc = ytn * c1;
     for (int k = 0; k < nd; ++k) {
         const int m = s_m[k];
         const int nst = nb - m + 1;  // starts 0..nb-m
@@ -1272,8 +1297,22 @@ __device__ __forceinline__ void box_scan_pad(Score &best, const double *Cw,
         const int *pn = Cn + s0, *pnm = Cn + s0 + m;
         const double *pw = Cw + s0, *pwm = Cw + s0 + m;
         const double *py = Cy + s0, *pym = Cy + s0 + m;
-        const int i0 = k * nval + s0;
+        const int i0 = (k << 16) + s0;
         const int trips = q / 64;
+        if (CFORM) {
+#pragma unroll 4
+            for (int i = 0; i < trips; ++i) {
+                const int o = i * 64;
+                const int ni = pnm[o] - pn[o];
+                const double n = (double)ni;
+                const double yin = pym[o] - py[o];
+                const double wout = fma(-c1, n, 1.0);
+                const double num = fma(ytc, n, -yin);
+                best.offer_later(ni < ntot, num * fabs(num),
+                                 LIKE ? n * wout * wout : n * wout, i0 + o);
+            }
+            continue;
+        }
 #pragma unroll 4
         for (int i = 0; i < trips; ++i) {
             const int o = i * 64;
@@ -1364,8 +1403,8 @@ __device__ __forceinline__ int scan_overhang(int nb, int mmax) {
 
 
 template <bool LIKE>
-__device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
-                              long long p, double period, int nb, int base,
+__device__ __forceinline__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
+                              long long p, int nb, int base,
                               long long *binW, long long *binY, int *cnt,
                               const short *s_m, int nd, int mmax,
                               long long *lscr, double *dscr, int *iscr) {
@@ -1437,7 +1476,7 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
             const long long ys =
                 (long long)(((unsigned long long)(unsigned)yhi << 32) |
                             (unsigned)ow);
-            Cy[i] = (double)ys * fy;
+            Cy[i] = small_i64_to_f64(ys) * fy;
         } else {
             oy += binY[base + i];
             Cw[i] = (double)ow * fw;
@@ -1465,21 +1504,26 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
     const double Wt = P.W_tot, Yt = P.Y_tot;
     const double ytn = Yt / Wt, epsn = DBL_EPSILON / Wt;
     Score best;
-    if (packed || counted)
-        box_scan_pad<true, LIKE>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn, epsn);
+    if ((packed || counted) && c1 >= epsn)
+        box_scan_pad<true, LIKE, true>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn,
+                                       epsn, P.n);
+    else if (packed || counted)
+        box_scan_pad<true, LIKE, false>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn,
+                                        epsn, 0);
     else
-        box_scan_pad<false, LIKE>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn,
-                                  epsn);
+        box_scan_pad<false, LIKE, false>(best, Cw, Cn, Cy, c1, s_m, nd, nb,
+                                         ytn, epsn, 0);
     // workgroup argmax (highest score, then lowest (k, n))
     int bi = block_argmax(best, dscr, iscr);
     if (bi == INT_MAX) {  // no window of positive depth: the oracle's choice
                           // among those of depth 0, if any
         Score zero;
         if (packed || counted)
-            box_scan<true, LIKE>(zero, Cw, Cn, Cy, c1, s_m, nd, nb, ytn, epsn);
+            box_scan<true, LIKE>(zero, Cw, Cn, Cy, c1, s_m, nd, nb, ytn, epsn,
+                                 c1 >= epsn ? P.n : 0);
         else
             box_scan<false, LIKE>(zero, Cw, Cn, Cy, c1, s_m, nd, nb, ytn,
-                                  epsn);
+                                  epsn, 0);
         bi = block_argmax(zero, dscr, iscr);
     }
     if (tid == 0) {
@@ -1495,7 +1539,7 @@ __device__ void finish_period(const Batch &b, const Out &o, const LcParams &P,
             // the winner's window sums; k_fields derives the seven fields
             // from them in parallel over periods (the duration slot carries
             // -(index + 1) until then)
-            const int k = bi / nval, s = bi - k * nval, m = s_m[k];
+            const int k = bi >> 16, s = bi & 0xffff, m = s_m[k];
             o.depth[p] = ((packed || counted) ? (double)(Cn[s + m] - Cn[s]) * c1
                                               : Cw[s + m] - Cw[s]) *
                          Wt;
@@ -1952,6 +1996,21 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
 // lds_bins <= 2528 share a CU's 160 KiB of LDS.
 // EXACT: parity mode's fold (fast mode's otherwise); LIKE: the likelihood
 // objective (the snr otherwise).
+// Zeroes n bytes of LDS at a (a multiple of 4), 16 bytes per store where a
+// is 16-byte aligned; the workgroup's threads share the stores.
+__device__ __forceinline__ void lds_zero(void *a, int n) {
+    unsigned char *c = (unsigned char *)a;
+    int done = 0;
+    if (((unsigned)(size_t)c & 15u) == 0) {
+        const int n16 = n >> 4;
+        for (int i = threadIdx.x; i < n16; i += NT)
+            ((int4 *)c)[i] = make_int4(0, 0, 0, 0);
+        done = n16 << 4;
+    }
+    for (int i = done + 4 * (int)threadIdx.x; i < n; i += 4 * NT)
+        *(int *)(c + i) = 0;
+}
+
 template <bool EXACT, bool LIKE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4)))
 void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
@@ -2030,6 +2089,19 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
         const double bd = P.bd;
         const double inv_bd = 1.0 / bd;
         const int os = b.oversample;
+        // lane j of each wave prepares the item's period pbeg + j (an item
+        // holds at most 64): its bins, and the exact fold's P/bd and 1/P
+        // scaled up by 1 + 2^-48 (see bin_points); the passes read them
+        // back as uniform values
+        const int lj = tid & 63;
+        double per_l = 1.0, inv_l = 1.0;
+        int nb_l = 0;
+        if (pbeg + lj < pend) {
+            const double ps = b.periods[pbeg + lj];
+            nb_l = (int)ceil(ps / bd) + os;
+            per_l = EXACT ? ps * inv_bd : ps;
+            inv_l = EXACT ? (1.0 / ps) * (1.0 + 0x1p-48) : 1.0 / ps;
+        }
 
         long long p = pbeg;
         while (p < pend) {
@@ -2045,6 +2117,8 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
             const int cap2 = counted_cap(lds_bins);
             const int cap = packed ? 2 * lds_bins : (counted ? cap2 : lds_bins);
             int *cntb = (int *)(binW + cap2);
+            // the pass's slot s is the item's period l0 + s (lane l0 + s)
+            const int l0 = (int)(p - pbeg);
             double per[SUBMAX], inv[SUBMAX];
             int nb[SUBMAX], base[SUBMAX];
             int cnt = 0, used = 0;
@@ -2058,8 +2132,7 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
 #pragma unroll
             for (int s = 0; s < SUBMAX; ++s) {
                 if (cnt == s && p + s < pend) {
-                    const double ps = b.periods[p + s];
-                    const int nbs = (int)ceil(ps / bd) + os;
+                    const int nbs = __builtin_amdgcn_readlane(nb_l, l0 + s);
                     // the period on top also needs room for its box-scan
                     // sentinels: the overhang after its point counts
                     // (packed) or weights (unpacked)
@@ -2067,11 +2140,8 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
                     if (used + (nbs + 1) +
                             (packed ? max((nbs + 1 + ov + 1) / 2, ov) : ov) <=
                         cap) {
-                        // the exact fold takes P/bd and 1/P scaled up by
-                        // 1 + 2^-48 (see bin_points)
-                        per[s] = EXACT ? ps * inv_bd : ps;
-                        inv[s] = EXACT ? (1.0 / ps) * (1.0 + 0x1p-48)
-                                       : 1.0 / ps;
+                        per[s] = readlane_d(per_l, l0 + s);
+                        inv[s] = readlane_d(inv_l, l0 + s);
                         nb[s] = nbs;
                         base[s] = used;
                         used += nbs + 1;
@@ -2093,11 +2163,9 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
                 ++p;
                 continue;
             }
-            for (int i = tid; i < used; i += NT) {
-                binW[i] = 0;
-                if (counted) cntb[i] = 0;
-                else if (!packed) binY[i] = 0;
-            }
+            lds_zero(binW, used * 8);
+            if (counted) lds_zero(cntb, used * 4);
+            else if (!packed) lds_zero(binY, used * 8);
             __syncthreads();
 
             // fold + binning over the lane's points: the lane-major layout
@@ -2136,10 +2204,9 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
             if (packed) {
                 int end = used;
                 for (int s = cnt - 1; s >= 0; --s) {
-                    const double ps = b.periods[p + s];
-                    const int nbs = (int)ceil(ps / bd) + os;
+                    const int nbs = __builtin_amdgcn_readlane(nb_l, l0 + s);
                     const int bse = end - (nbs + 1);
-                    finish_period<LIKE>(b, o, P, p + s, ps, nbs, bse, binW, binW,
+                    finish_period<LIKE>(b, o, P, p + s, nbs, bse, binW, binW,
                                         nullptr, s_m, nd, mmax, lscr, dscr,
                                         iscr);
                     end = bse;
@@ -2149,10 +2216,9 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
                 // periods already finished
                 int end = used;
                 for (int s = cnt - 1; s >= 0; --s) {
-                    const double ps = b.periods[p + s];
-                    const int nbs = (int)ceil(ps / bd) + os;
+                    const int nbs = __builtin_amdgcn_readlane(nb_l, l0 + s);
                     const int bse = end - (nbs + 1);
-                    finish_period<LIKE>(b, o, P, p + s, ps, nbs, bse, binW, binY,
+                    finish_period<LIKE>(b, o, P, p + s, nbs, bse, binW, binY,
                                         counted ? cntb : nullptr, s_m, nd, mmax,
                                         lscr, dscr, iscr);
                     end = bse;
@@ -2247,8 +2313,7 @@ __global__ __launch_bounds__(NT) void k_fields(Batch b, Out o, Work w) {
         if (!(mark < 0.0)) continue;
         const int bi = (int)(-mark - 1.0);
         const double period = b.periods[p];
-        const int nval = (int)ceil(period / P.bd) + b.oversample + 1;
-        const int k = bi / nval, s = bi - k * nval;
+        const int k = bi >> 16, s = bi & 0xffff;  // k_main's window index
         const int m =
             (short)fmin(round(b.durs[dof + k] / P.bd), 32767.0);  // as k_main
         const double win = o.depth[p], yin_s = o.depth_err[p];
@@ -3603,6 +3668,8 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
     // passes over each curve's points) when the batch has periods to spare
     // for eight items per workgroup, short ones for a small batch, so that
     // its few curves still spread over the whole GPU
+    // at most 64 (set_tuning checks it): k_main holds an item's periods one
+    // per lane of a wave
     const int per_item =
         c->tune.periods_per_item > 0
             ? c->tune.periods_per_item
