@@ -438,7 +438,9 @@ __device__ __forceinline__ void bin_points(
             if (!PACKED) yv_n = load_at<long long>(by, lo);
         }
         int at[C];
-        bool fl[C];
+        // any period's quotient near an integer; the flags themselves are
+        // recomputed on the rare path below rather than held through the
+        // fold (C lane masks of scalar registers, which then spill)
         bool any = false;
         // t'/bd + tol_x, shared by the C periods of the pass (one rounding;
         // within the tolerance analysis below, which allows two)
@@ -465,8 +467,7 @@ __device__ __forceinline__ void bin_points(
                 const double q = floor(tv * inv[s]);
                 const double x = fma(q, -per[s], tb);
                 const double fx = __builtin_amdgcn_fract(x);
-                fl[s] = fx <= tol2;
-                any |= fl[s];
+                any |= fx <= tol2;
                 at[s] = (int)x;  // bin at[s] + 1
             } else {
                 // fast mode: FMA phase and rounded reciprocals only; a point
@@ -508,12 +509,13 @@ __device__ __forceinline__ void bin_points(
             const double *pgx = pg;
             asm volatile("" : "+s"(pgx));
 #pragma unroll
-            for (int s = 0; s < C; ++s)
-                if (fl[s]) {
+            for (int s = 0; s < C; ++s) {
+                // the main path's fold again, and its flag
+                const double xs =
+                    fma(floor(tx * inv[s]), -per[s], fma(tx, inv_bd, tol_x));
+                if (__builtin_amdgcn_fract(xs) <= tol2) {
                     const double Ps = pgx[s];
-                    const int a = base[s] + 1 +
-                                  (int)fma(floor(tx * inv[s]), -per[s],
-                                           fma(tx, inv_bd, tol_x));
+                    const int a = base[s] + 1 + (int)xs;
                     const double ph = tx - Ps * floor(tx / Ps);
                     const int e =
                         base[s] + min(max((int)trunc(ph / bd) + 1, 0), nb[s]);
@@ -543,6 +545,7 @@ __device__ __forceinline__ void bin_points(
                         }
                     }
                 }
+            }
         }
     }
     if (UW && npad > 0 && threadIdx.x == 0) {
