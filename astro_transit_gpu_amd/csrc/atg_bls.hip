@@ -130,9 +130,14 @@ struct Work {
     long long *ws_w;      // lane-major quantized ivar (packed curves: the
                           // bin increment (1 << ybits) + ws_y, 0 for padding)
     long long *ws_y;      // lane-major quantized ivar * (y - median)
-    double *nat_t;        // the same three arrays in point order, written
-    long long *nat_w;     // only for time-sorted curves with periods whose
-    long long *nat_y;     // bins exceed the LDS histogram
+    // the same values in point order, written only for time-sorted curves
+    // with periods whose bins exceed the LDS histogram: t' and the
+    // quantized ivar * (y - median) side by side in 16-byte records (slot
+    // i: nat_ty[2i] = t', nat_ty[2i + 1] = its bits), so that one 16-byte
+    // load brings a point; the quantized ivar apart (read only for
+    // varying weights)
+    double *nat_ty;
+    long long *nat_w;
     unsigned int *cand_count;  // periods queued for refinement
     unsigned int *cand_next;   // refinement queue head
     // periods left to k_big (bins exceed LDS), in NQ lists, one per
@@ -356,7 +361,7 @@ __device__ int lower_key(const double *tn, int n, double P, double invP,
     int lo = 0, hi = n;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        const double t = tn[nat_slot(mid)];
+        const double t = tn[2 * nat_slot(mid)];
         const double q = fold_cycle(t, P, invP, fast);
         const bool less =
             q < c ||
@@ -377,6 +382,9 @@ __device__ __forceinline__ gchar *scalar_base(const void *p) {
     const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
     return (gchar *)(((unsigned long long)hi << 32) | lo);
 }
+// a 16-byte record of two int64, one 16-byte load (byte_off a multiple of
+// 16 from a 16-byte aligned base)
+typedef long long I64x2 __attribute__((ext_vector_type(2)));
 template <typename T>
 __device__ __forceinline__ T load_at(gchar *base, unsigned byte_off) {
     return *(const __attribute__((address_space(1))) T *)(base + byte_off);
@@ -1068,9 +1076,9 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
                     tv = tr[u] - tmin;
                 }
                 const long long sl = nbase + nat_slot(i);
-                w.nat_t[sl] = tv;
+                ((longlong2 *)w.nat_ty)[sl] =
+                    make_longlong2(__double_as_longlong(tv), yq);
                 w.nat_w[sl] = wq;
-                w.nat_y[sl] = yq;
             }
         }
     }
@@ -1603,13 +1611,16 @@ __device__ __forceinline__ void seg_add(int j, int s0, int hi, long long wv,
 // loaded (16 instead of 24 bytes read per point and period).
 template <bool FAST, bool PAD, bool UW>
 __device__ __forceinline__ void big_fold_range(
-    const double *tn, const long long *wn, const long long *yn, int a, int e,
+    const double *tn, const long long *wn, int a, int e,
     double period, double qc, double bd, double inv_bd, double tol_x, int nb,
     int os, int s0, int hi, long long *bins, long long wone) {
     if (a >= e) return;
     const unsigned tid = threadIdx.x;
     const int pt = (tid & 63) * NW + (tid >> 6);  // point of slot tid
-    gchar *tb = scalar_base(tn), *wb = scalar_base(wn), *yb = scalar_base(yn);
+    // tn: the interleaved (t', y) records, one 16-byte load a point, at a
+    // 32-bit byte offset (the ranged form takes curves of fewer than 2^27
+    // points); wn: the weights (8 bytes a point)
+    gchar *tb = scalar_base(tn), *wb = scalar_base(wn);
     const unsigned span = (unsigned)(hi - s0), cnt = (unsigned)(e - a);
     const double pq = period * qc;
     // Parity mode: x = t * inv_bd + (tol_x - pq * inv_bd), one FMA, is within
@@ -1637,9 +1648,10 @@ __device__ __forceinline__ void big_fold_range(
     // masked: test the point index (partial blocks); PAD also the segment
     auto fold_block = [&](int B, bool masked) {
         const unsigned o = ((unsigned)B + tid) * 8u;
-        const double t = load_at<double>(tb, o);
+        const I64x2 ty = load_at<I64x2>(tb, 2u * o);
+        const double t = __longlong_as_double(ty.x);
         const long long wv = UW ? wone : load_at<long long>(wb, o);
-        const long long yv = load_at<long long>(yb, o);
+        const long long yv = ty.y;
         bool fl;
         int r = rough(t, fl);
         if (fl) r = exact(t);
@@ -1662,9 +1674,10 @@ __device__ __forceinline__ void big_fold_range(
         long long wv[4], yv[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            t[u] = load_at<double>(tb, o + u * NT * 8);
+            const I64x2 ty = load_at<I64x2>(tb, 2u * o + u * NT * 16);
+            t[u] = __longlong_as_double(ty.x);
             wv[u] = UW ? wone : load_at<long long>(wb, o + u * NT * 8);
-            yv[u] = load_at<long long>(yb, o + u * NT * 8);
+            yv[u] = ty.y;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -1743,12 +1756,13 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
     const double c1 = (double)P.wq_one * (P.w_unit / P.W_tot);
     // ranged form: cycles 0..Q of the sorted points
     const long long nbase = nat_base(b, P.lc_self);
-    const double *tn = w.nat_t + nbase;
+    const double *tn = w.nat_ty + 2 * nbase;  // interleaved (t', y) records
     int ncyc = 0;
     if (P.natural)
-        ncyc = (int)fold_cycle(tn[nat_slot(P.n - 1)], period, invP, fastm) + 1;
+        ncyc = (int)fold_cycle(tn[2 * nat_slot(P.n - 1)], period, invP,
+                               fastm) + 1;
     const bool ranged = P.natural && ncyc <= (1 << 20) &&
-                        P.n < (1 << 28) &&
+                        P.n < (1 << 27) &&
                         lds_bins - ncyc - mmax - 1 >= 64;
     int S = ranged ? lds_bins - ncyc - mmax - 1 : lds_bins - mmax - 1;
     // The range table takes R 16-byte bins at the tail of the tile.  Where
@@ -1773,7 +1787,6 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
     int *tab = (int *)(bins + 2 * (lds_bins - R));
     int *erng = tab + 2 * ncyc * (batched ? nseg : 1);
     const long long *wn = w.nat_w + nbase;
-    const long long *yn = w.nat_y + nbase;
     Score best;
     double bwin = 0.0, byin = 0.0;
     const double *tp = w.ws_t + P.ws_off + tid;
@@ -1825,7 +1838,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
             // direct destinations: bins [s0, hi] of every cycle; then the
             // padding copies, bins 1..os of every cycle to nb-os..nb-1
 #define ATG_RANGE(F, PD, U, J0, E)                                           \
-    big_fold_range<F, PD, U>(tn, wn, yn, J0, E, period, (double)c, bd,       \
+    big_fold_range<F, PD, U>(tn, wn, J0, E, period, (double)c, bd,           \
                              inv_bd, P.tol_x, nb, os, s0, hi, bins, P.wq_one)
 #define ATG_RANGES(PD, TAB)                                                  \
     for (int c = 0; c < ncyc; ++c) {                                         \
@@ -2540,10 +2553,10 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
     // of bins 1..os whose copies land in the segment.  Each bin still
     // receives its points in point order.
     const bool fits = nb + 1 <= lds_bins;
-    const double *tn = w.nat_t + nat_base(b, lc);
+    const double *tn = w.nat_ty + 2 * nat_base(b, lc);
     int ncyc = 0;
     if (!fits && P.natural)
-        ncyc = (int)fold_cycle(tn[nat_slot(n - 1)], period, invP) + 1;
+        ncyc = (int)fold_cycle(tn[2 * nat_slot(n - 1)], period, invP) + 1;
     const int T = 2 * ncyc + 2;  // tail slots of my[] for the range table
     const bool ranged = !fits && P.natural && nb - os > os &&
                         ncyc <= (1 << 20) && lds_bins - T - mmax - 1 >= 64;
@@ -3468,7 +3481,7 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct Layout {
     size_t prm, flags, items, queues, counter, cand_lc, cand_p, t, w, y,
-        nt, nw, ny, tk_pos, tk_st, big_q, big_p, big_lc, total;
+        nt, nw, tk_pos, tk_st, big_q, big_p, big_lc, total;
 };
 
 Layout layout_of(const atg_bls_batch *b) {
@@ -3489,9 +3502,8 @@ Layout layout_of(const atg_bls_batch *b) {
     L.w = o;       o = align256(o + slots * sizeof(long long));
     L.y = o;       o = align256(o + slots * sizeof(long long));
     const size_t npt = (size_t)b->n_points + nlc * NT;  // whole blocks
-    L.nt = o;      o = align256(o + npt * sizeof(double));
+    L.nt = o;      o = align256(o + npt * 2 * sizeof(double));
     L.nw = o;      o = align256(o + npt * sizeof(long long));
-    L.ny = o;      o = align256(o + npt * sizeof(long long));
     L.tk_pos = o;  o = align256(o + nper * sizeof(int));
     L.tk_st = o;   o = align256(o + nper);
     L.big_p = o;   o = align256(o + nper * sizeof(long long));
@@ -3615,9 +3627,8 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
     w.ws_t = (double *)(base + L.t);
     w.ws_w = (long long *)(base + L.w);
     w.ws_y = (long long *)(base + L.y);
-    w.nat_t = (double *)(base + L.nt);
+    w.nat_ty = (double *)(base + L.nt);
     w.nat_w = (long long *)(base + L.nw);
-    w.nat_y = (long long *)(base + L.ny);
     w.cand_count = w.counter + 1;
     w.cand_next = w.counter + 2;
     w.big_base = (long long *)(base + L.big_q);
