@@ -1607,8 +1607,9 @@ __device__ __forceinline__ void seg_add(int j, int s0, int hi, long long wv,
 // is one the padding assignment overwrites (nb - os .. nb - 1) is added
 // like the others; big_period clears those bins before the copies go in.
 // (the curve's padded copy must span less than 2^32 bytes)
-// UW: every point of the curve has the weight wone, which is then not
-// loaded (16 instead of 24 bytes read per point and period).
+// UW: every point of the curve has the same weight; it adds wone (1: the
+// weight channel counts points) without a load (16 instead of 24 bytes
+// read per point and period).
 template <bool FAST, bool PAD, bool UW>
 __device__ __forceinline__ void big_fold_range(
     const double *tn, const long long *wn, int a, int e,
@@ -1698,13 +1699,18 @@ __device__ __forceinline__ void big_fold_range(
 // of bin s0 + l (uniform weights: the point count as weight, times c1), the
 // thread taking starts s0 + tid, s0 + tid + NT, ...  sb receives the
 // thread's best window (the first of equal scores: its indices increase).
-template <bool PACKED, bool LIKE>
+template <bool PACKED, bool LIKE, bool CFORM>
 __device__ __forceinline__ void big_box_scan(Score &sb, const double2 *C2,
                                              int s0, int nend, int nb,
                                              const int *s_m, int nd,
                                              double c1, double ytn,
-                                             double epsn) {
+                                             double epsn, int ntot) {
     const int nval = nb + 1;
+    // CFORM (uniform weights, one point's weight c1 at least epsn): the
+    // weight tests are 0 < n < ntot on the window's point count n, and c1
+    // moves into the constants, as in k_main's box_scan_pad (the same
+    // scores, so both kernels rank a period's windows alike)
+    const double ytc = ytn * c1, ntop = (double)ntot - 0.5;
     for (int k = 0; k < nd; ++k) {
         const int m = s_m[k];
         const int last = min(nend - 1, nb - m);
@@ -1712,8 +1718,18 @@ __device__ __forceinline__ void big_box_scan(Score &sb, const double2 *C2,
         for (int n = s0 + (int)threadIdx.x; n <= last; n += NT) {
             const int l = n - s0;
             const double2 c0 = C2[l], ce = C2[l + m];
-            const double win = PACKED ? (ce.x - c0.x) * c1 : ce.x - c0.x;
             const double yin = ce.y - c0.y;
+            if (CFORM) {
+                const double cnt = ce.x - c0.x;
+                const double wout = fma(-c1, cnt, 1.0);
+                const double num = fma(ytc, cnt, -yin);
+                sb.offer_later((cnt > 0.5) & (cnt < ntop) & !(num < 0.0),
+                               num * num,
+                               LIKE ? cnt * wout * wout : cnt * wout,
+                               k * nval + n);
+                continue;
+            }
+            const double win = PACKED ? (ce.x - c0.x) * c1 : ce.x - c0.x;
             const double wout = 1.0 - win;
             const double num = fma(ytn, win, -yin);
             sb.offer_later(!(fmin(win, wout) < epsn) & !(num < 0.0),
@@ -1839,7 +1855,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
             // padding copies, bins 1..os of every cycle to nb-os..nb-1
 #define ATG_RANGE(F, PD, U, J0, E)                                           \
     big_fold_range<F, PD, U>(tn, wn, J0, E, period, (double)c, bd,           \
-                             inv_bd, P.tol_x, nb, os, s0, hi, bins, P.wq_one)
+                             inv_bd, P.tol_x, nb, os, s0, hi, bins, 1)
 #define ATG_RANGES(PD, TAB)                                                  \
     for (int c = 0; c < ncyc; ++c) {                                         \
         const int j0 = TAB[2 * c], e = TAB[2 * c + 1];                       \
@@ -1871,7 +1887,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
             for (int i = 0; i < P.L; ++i) {
                 long long wv = wp[(long long)i * NT];
                 const long long yv = yp[(long long)i * NT];
-                if (P.packed) wv = wv != 0 ? P.wq_one : 0;
+                if (P.uniform) wv = wv != 0 ? 1 : 0;  // point counts
                 if ((wv | yv) == 0) continue;
                 const double tv = tp[(long long)i * NT];
                 const int r =
@@ -1908,10 +1924,10 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
                 s_carry[0] = ow;
                 s_carry[1] = oy;
             }
-            // uniform weights: the point count, as in finish_period (every
-            // point adds the common weight wq_one)
+            // uniform weights: the point count, as in finish_period (the
+            // weight channel counts points)
             double2 cv;
-            cv.x = P.uniform ? (double)(ow / P.wq_one) : (double)ow * fw;
+            cv.x = P.uniform ? (double)ow : (double)ow * fw;
             cv.y = (double)oy * fy;
             ((double2 *)C)[i] = cv;
         }
@@ -1923,13 +1939,19 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
         // rule; it joins the period's best with the full one (indices are
         // not monotone across segments)
         Score sb;
-        if (P.uniform) {
-            if (like) big_box_scan<true, true>(sb, C2, s0, nend, nb, s_m, nd, c1, ytn, epsn);
-            else big_box_scan<true, false>(sb, C2, s0, nend, nb, s_m, nd, c1, ytn, epsn);
+#define ATG_BBS(U, L, CF)                                                    \
+    big_box_scan<U, L, CF>(sb, C2, s0, nend, nb, s_m, nd, c1, ytn, epsn, P.n)
+        if (P.uniform && c1 >= epsn) {
+            if (like) ATG_BBS(true, true, true);
+            else ATG_BBS(true, false, true);
+        } else if (P.uniform) {
+            if (like) ATG_BBS(true, true, false);
+            else ATG_BBS(true, false, false);
         } else {
-            if (like) big_box_scan<false, true>(sb, C2, s0, nend, nb, s_m, nd, c1, ytn, epsn);
-            else big_box_scan<false, false>(sb, C2, s0, nend, nb, s_m, nd, c1, ytn, epsn);
+            if (like) ATG_BBS(false, true, false);
+            else ATG_BBS(false, false, false);
         }
+#undef ATG_BBS
         if (sb.idx != INT_MAX) {
             const int before = best.idx;
             best.offer(sb.q, sb.den, sb.idx);
