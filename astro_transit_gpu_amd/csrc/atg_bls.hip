@@ -239,7 +239,10 @@ __device__ __forceinline__ long long dpp_add_i64(long long v) {
 
 // Exclusive int64 scan over the workgroup in thread order, the wave part
 // with DPP lane moves (row shifts 1, 2, 4, 8, then row broadcasts 15 and
-// 31: an inclusive scan without LDS traffic); *total gets the sum.
+// 31: an inclusive scan without LDS traffic); *total gets the sum.  LEAD
+// false omits the barrier that keeps the scratch's previous readers ahead
+// of its new writers, for callers with a barrier since its last use.
+template <bool LEAD = true>
 __device__ __forceinline__ long long block_excl_scan_dpp(long long v,
                                                          long long *scratch,
                                                          long long *total) {
@@ -251,7 +254,7 @@ __device__ __forceinline__ long long block_excl_scan_dpp(long long v,
     inc = dpp_add_i64<0x118, 0xF>(inc);  // row_shr:8
     inc = dpp_add_i64<0x142, 0xA>(inc);  // row_bcast:15
     inc = dpp_add_i64<0x143, 0xC>(inc);  // row_bcast:31
-    __syncthreads();
+    if (LEAD) __syncthreads();
     if (l == 63) scratch[w] = inc;
     __syncthreads();
     long long off = 0, tot = 0;
@@ -1357,7 +1360,8 @@ __device__ __forceinline__ double dpp_d(double v) {
 // rest of the wave), and then the lowest index among the lanes holding it
 // the same way.  Both operations are commutative and associative, so the
 // result does not depend on the order of combination.  Returns the index
-// (INT_MAX: no window) in every thread.
+// (INT_MAX: no window) in every thread.  LEAD as in block_excl_scan_dpp.
+template <bool LEAD = true>
 __device__ __forceinline__ int block_argmax(const Score &best, double *dscr,
                                             int *iscr) {
     const double sc = best.idx == INT_MAX ? -1.0 : best.q / best.den;
@@ -1378,7 +1382,7 @@ __device__ __forceinline__ int block_argmax(const Score &best, double *dscr,
     c = min(c, __builtin_amdgcn_update_dpp(c, c, 0x143, 0xC, 0xF, false));
     const int ci = __builtin_amdgcn_readlane(c, 63);
     const int wv = threadIdx.x >> 6;
-    __syncthreads();
+    if (LEAD) __syncthreads();
     if ((threadIdx.x & 63) == 0) {
         dscr[wv] = mx;
         iscr[wv] = ci;
@@ -1454,8 +1458,11 @@ __device__ __forceinline__ void finish_period(const Batch &b, const Out &o, cons
         }
     }
     long long totw, toty;
-    long long ow = block_excl_scan_dpp(sw, lscr, &totw);
-    long long oy = packed ? 0 : block_excl_scan_dpp(sy, lscr, &toty);
+    // the barrier after the padding keeps every reader of the scratch (the
+    // previous period's) ahead; the second scan has its own scratch
+    long long ow = block_excl_scan_dpp<false>(sw, lscr, &totw);
+    long long oy = packed ? 0 : block_excl_scan_dpp<false>(sy, lscr + NW,
+                                                           &toty);
     // cumulative sums in units of the total weight (FP64).  Packed bins:
     // the flux prefix replaces the histogram in place and the weight prefix
     // is kept as the int32 point count Cn just above it (12 bytes a bin);
@@ -1525,7 +1532,9 @@ __device__ __forceinline__ void finish_period(const Batch &b, const Out &o, cons
         box_scan_pad<false, LIKE, false>(best, Cw, Cn, Cy, c1, s_m, nd, nb,
                                          ytn, epsn, 0);
     // workgroup argmax (highest score, then lowest (k, n))
-    int bi = block_argmax(best, dscr, iscr);
+    // the scratch's last readers (the previous period's argmax) are behind
+    // this period's barriers
+    int bi = block_argmax<false>(best, dscr, iscr);
     if (bi == INT_MAX) {  // no window of positive depth: the oracle's choice
                           // among those of depth 0, if any
         Score zero;
@@ -1558,7 +1567,9 @@ __device__ __forceinline__ void finish_period(const Batch &b, const Out &o, cons
             o.duration[p] = -(double)bi - 1.0;
         }
     }
-    __syncthreads();
+    // no barrier here: the next period's writes follow its padding barrier
+    // (which thread 0 reaches after these reads), and k_main puts one
+    // before the next pass clears the histogram
 }
 
 // Destinations of a point in raw bin r once the padding assignment
@@ -2061,7 +2072,7 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
     __shared__ short s_m[DMAX];
     __shared__ unsigned int s_item;
     __shared__ int s_queue;
-    __shared__ long long lscr[NW];
+    __shared__ long long lscr[2 * NW];  // two scans' scratch
     __shared__ double dscr[2 * NW];
     __shared__ int iscr[NW];
 
@@ -2201,6 +2212,7 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
                 ++p;
                 continue;
             }
+            __syncthreads();  // the previous pass's last reads
             lds_zero(binW, used * 8);
             if (counted) lds_zero(cntb, used * 4);
             else if (!packed) lds_zero(binY, used * 8);
