@@ -237,16 +237,9 @@ __device__ __forceinline__ long long dpp_add_i64(long long v) {
     return v + (long long)(((unsigned long long)h2 << 32) | l2);
 }
 
-// Exclusive int64 scan over the workgroup in thread order, the wave part
-// with DPP lane moves (row shifts 1, 2, 4, 8, then row broadcasts 15 and
-// 31: an inclusive scan without LDS traffic); *total gets the sum.  LEAD
-// false omits the barrier that keeps the scratch's previous readers ahead
-// of its new writers, for callers with a barrier since its last use.
-template <bool LEAD = true>
-__device__ __forceinline__ long long block_excl_scan_dpp(long long v,
-                                                         long long *scratch,
-                                                         long long *total) {
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+// Inclusive int64 scan over the wave in lane order with DPP lane moves (row
+// shifts 1, 2, 4, 8, then row broadcasts 15 and 31), no LDS traffic.
+__device__ __forceinline__ long long wave_incl_scan_dpp(long long v) {
     long long inc = v;
     inc = dpp_add_i64<0x111, 0xF>(inc);  // row_shr:1
     inc = dpp_add_i64<0x112, 0xF>(inc);  // row_shr:2
@@ -254,6 +247,28 @@ __device__ __forceinline__ long long block_excl_scan_dpp(long long v,
     inc = dpp_add_i64<0x118, 0xF>(inc);  // row_shr:8
     inc = dpp_add_i64<0x142, 0xA>(inc);  // row_bcast:15
     inc = dpp_add_i64<0x143, 0xC>(inc);  // row_bcast:31
+    return inc;
+}
+
+// Orders one wave's LDS accesses before its later ones, across lanes (the
+// wave's LDS operations complete in order; this keeps the compiler from
+// moving them across).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Exclusive int64 scan over the workgroup in thread order, the wave part
+// wave_incl_scan_dpp; *total gets the sum.  LEAD
+// false omits the barrier that keeps the scratch's previous readers ahead
+// of its new writers, for callers with a barrier since its last use.
+template <bool LEAD = true>
+__device__ __forceinline__ long long block_excl_scan_dpp(long long v,
+                                                         long long *scratch,
+                                                         long long *total) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const long long inc = wave_incl_scan_dpp(v);
     if (LEAD) __syncthreads();
     if (l == 63) scratch[w] = inc;
     __syncthreads();
@@ -1359,11 +1374,11 @@ __device__ __forceinline__ double dpp_d(double v) {
 // and 31: every lane combines with lanes whose partial results cover the
 // rest of the wave), and then the lowest index among the lanes holding it
 // the same way.  Both operations are commutative and associative, so the
-// result does not depend on the order of combination.  Returns the index
-// (INT_MAX: no window) in every thread.  LEAD as in block_excl_scan_dpp.
-template <bool LEAD = true>
-__device__ __forceinline__ int block_argmax(const Score &best, double *dscr,
-                                            int *iscr) {
+// result does not depend on the order of combination.  wave_argmax
+// returns the wave's (score, index) in every lane; block_argmax returns the
+// workgroup's index (INT_MAX: no window) in every thread (LEAD as in
+// block_excl_scan_dpp).
+__device__ __forceinline__ int wave_argmax(const Score &best, double &mx) {
     const double sc = best.idx == INT_MAX ? -1.0 : best.q / best.den;
     double v = sc;
     v = fmax(v, dpp_d<0xB1, 0xF>(v));   // quad_perm [1,0,3,2]
@@ -1372,7 +1387,7 @@ __device__ __forceinline__ int block_argmax(const Score &best, double *dscr,
     v = fmax(v, dpp_d<0x140, 0xF>(v));  // row_mirror
     v = fmax(v, dpp_d<0x142, 0xA>(v));  // row_bcast:15
     v = fmax(v, dpp_d<0x143, 0xC>(v));  // row_bcast:31
-    const double mx = readlane_d(v, 63);
+    mx = readlane_d(v, 63);
     int c = sc == mx ? best.idx : INT_MAX;
     c = min(c, __builtin_amdgcn_update_dpp(c, c, 0xB1, 0xF, 0xF, false));
     c = min(c, __builtin_amdgcn_update_dpp(c, c, 0x4E, 0xF, 0xF, false));
@@ -1380,7 +1395,13 @@ __device__ __forceinline__ int block_argmax(const Score &best, double *dscr,
     c = min(c, __builtin_amdgcn_update_dpp(c, c, 0x140, 0xF, 0xF, false));
     c = min(c, __builtin_amdgcn_update_dpp(c, c, 0x142, 0xA, 0xF, false));
     c = min(c, __builtin_amdgcn_update_dpp(c, c, 0x143, 0xC, 0xF, false));
-    const int ci = __builtin_amdgcn_readlane(c, 63);
+    return __builtin_amdgcn_readlane(c, 63);
+}
+template <bool LEAD = true>
+__device__ __forceinline__ int block_argmax(const Score &best, double *dscr,
+                                            int *iscr) {
+    double mx;
+    const int ci = wave_argmax(best, mx);
     const int wv = threadIdx.x >> 6;
     if (LEAD) __syncthreads();
     if ((threadIdx.x & 63) == 0) {
@@ -1570,6 +1591,196 @@ __device__ __forceinline__ void finish_period(const Batch &b, const Out &o, cons
     // no barrier here: the next period's writes follow its padding barrier
     // (which thread 0 reaches after these reads), and k_main puts one
     // before the next pass clears the histogram
+}
+
+// Box scans of one period by one wave (finish_period_wave): the starts of
+// each duration 64 at a time, one per lane, the lanes past the last start
+// of the final stride repeating it (same score and index).  The window
+// arithmetic, scores and tests are box_scan_pad's (positive depth) and
+// box_window's (the depth-0 rescan).
+template <bool PACKED, bool LIKE, bool CFORM>
+__device__ __forceinline__ void wave_box_scan(Score &best, const double *Cw,
+                                              const int *Cn, const double *Cy,
+                                              double c1, const short *s_m,
+                                              int nd, int nb, double ytn,
+                                              double epsn, int ntot) {
+    const int lane = (int)threadIdx.x & 63;
+    best.q = 0.0;
+    const double ytc = ytn * c1;
+    auto window = [&](int st, int m, int idx) {
+        if (CFORM) {
+            const int ni = Cn[st + m] - Cn[st];
+            const double n = (double)ni;
+            const double yin = Cy[st + m] - Cy[st];
+            const double wout = fma(-c1, n, 1.0);
+            const double num = fma(ytc, n, -yin);
+            best.offer_later(ni < ntot, num * fabs(num),
+                             LIKE ? n * wout * wout : n * wout, idx);
+        } else {
+            const double win = PACKED ? (double)(Cn[st + m] - Cn[st]) * c1
+                                      : Cw[st + m] - Cw[st];
+            const double yin = Cy[st + m] - Cy[st];
+            const double wout = 1.0 - win;
+            const double num = fma(ytn, win, -yin);
+            best.offer_later(!(fmin(win, wout) < epsn), num * fabs(num),
+                             LIKE ? win * wout * wout : win * wout, idx);
+        }
+    };
+    for (int k = 0; k < nd; ++k) {
+        const int m = s_m[k];
+        const int nst = nb - m + 1;  // starts 0..nb-m
+        if (nst <= 0) continue;
+        const int full = nst >> 6, kb = k << 16;
+        for (int i = 0; i < full; ++i) {
+            const int st = (i << 6) + lane;
+            window(st, m, kb + st);
+        }
+        if (nst & 63) {
+            const int st = min((full << 6) + lane, nst - 1);
+            window(st, m, kb + st);
+        }
+    }
+}
+
+template <bool PACKED, bool LIKE>
+__device__ __forceinline__ void wave_box_rescan(Score &best, const double *Cw,
+                                                const int *Cn, const double *Cy,
+                                                double c1, const short *s_m,
+                                                int nd, int nb, double ytn,
+                                                double epsn, int ntot) {
+    const int lane = (int)threadIdx.x & 63;
+    for (int k = 0; k < nd; ++k) {
+        const int m = s_m[k];
+        const int send = min(nb - 1, nb - m);
+        for (int st = lane; st - lane <= send; st += 64)
+            box_window<PACKED, LIKE>(best, Cw, Cn, Cy, c1, ytn, epsn, ntot,
+                                     min(st, send), m, (k << 16) + st,
+                                     st <= send);
+    }
+}
+
+// finish_period by one wave, with no workgroup barrier: k_main's passes of
+// short periods (four or more to a pass) give every period the space for
+// its own point counts, so that the four waves finish different periods
+// side by side.  The padding, the exact int64 prefix (a wave scan), the
+// FP64 conversion, the box scan and the argmax are finish_period's, in the
+// same order within the period.
+
+template <bool LIKE>
+__device__ __forceinline__ void finish_period_wave(
+    const Batch &b, const Out &o, const LcParams &P, long long p, int nb,
+    int base, long long *binW, long long *binY, int *cnt, const short *s_m,
+    int nd) {
+    const int lane = (int)threadIdx.x & 63;
+    const int os = b.oversample;
+    const bool packed = P.packed;
+    const bool counted = cnt != nullptr;
+    // padding: bins[nb-os+j-1] = bins[j], j = 1..os (an assignment)
+    if (nb - os > os) {
+        for (int j = 1 + lane; j <= os; j += 64) {
+            binW[base + nb - os + j - 1] = binW[base + j];
+            if (counted) cnt[base + nb - os + j - 1] = cnt[base + j];
+            else if (!packed) binY[base + nb - os + j - 1] = binY[base + j];
+        }
+    } else if (lane == 0) {  // overlapping ranges: the oracle's order
+        for (int j = 1, d = nb - os; j <= os; ++j, ++d) {
+            binW[base + d] = binW[base + j];
+            if (counted) cnt[base + d] = cnt[base + j];
+            else if (!packed) binY[base + d] = binY[base + j];
+        }
+    }
+    wave_lds_sync();
+
+    const int nval = nb + 1;
+    const int seg = (nval + 63) >> 6;
+    const int b0 = min(lane * seg, nval), b1 = min(b0 + seg, nval);
+    long long sw = 0, sy = 0;
+    for (int i = b0; i < b1; ++i) {
+        if (counted) {
+            sw += cnt[base + i];
+            sy += binW[base + i];
+        } else {
+            sw += binW[base + i];
+            if (!packed) sy += binY[base + i];
+        }
+    }
+    long long ow = wave_incl_scan_dpp(sw) - sw;
+    long long oy = packed ? 0 : wave_incl_scan_dpp(sy) - sy;
+    const double fw = P.w_unit / P.W_tot, fy = P.y_unit / P.W_tot;
+    const double c1 = (double)P.wq_one * fw;
+    double *Cw = (double *)binW + base;
+    double *Cy = (packed || counted) ? (double *)binW + base
+                                     : (double *)binY + base;
+    int *Cn = counted ? cnt + base : (int *)(binW + base + nval);
+    const int hb = P.ybits - 32;
+    for (int i = b0; i < b1; ++i) {
+        if (counted) {
+            ow += cnt[base + i];
+            oy += binW[base + i];
+            Cn[i] = (int)ow;
+            Cy[i] = (double)oy * fy;
+            continue;
+        }
+        ow += binW[base + i];
+        if (packed) {
+            const int hi = (int)((unsigned long long)ow >> 32);
+            const int yhi = (int)((unsigned)hi << (32 - hb)) >> (32 - hb);
+            Cn[i] = (int)((unsigned)(hi - yhi) >> hb);
+            const long long ys =
+                (long long)(((unsigned long long)(unsigned)yhi << 32) |
+                            (unsigned)ow);
+            Cy[i] = small_i64_to_f64(ys) * fy;
+        } else {
+            oy += binY[base + i];
+            Cw[i] = (double)ow * fw;
+            Cy[i] = (double)oy * fy;
+        }
+    }
+    wave_lds_sync();
+
+    const double Wt = P.W_tot, Yt = P.Y_tot;
+    const double ytn = Yt / Wt, epsn = DBL_EPSILON / Wt;
+    Score best;
+    if ((packed || counted) && c1 >= epsn)
+        wave_box_scan<true, LIKE, true>(best, Cw, Cn, Cy, c1, s_m, nd, nb, ytn,
+                                        epsn, P.n);
+    else if (packed || counted)
+        wave_box_scan<true, LIKE, false>(best, Cw, Cn, Cy, c1, s_m, nd, nb,
+                                         ytn, epsn, 0);
+    else
+        wave_box_scan<false, LIKE, false>(best, Cw, Cn, Cy, c1, s_m, nd, nb,
+                                          ytn, epsn, 0);
+    double mx;
+    int bi = wave_argmax(best, mx);
+    if (bi == INT_MAX) {  // no window of positive depth: the depth-0 rule
+        Score zero;
+        if (packed || counted)
+            wave_box_rescan<true, LIKE>(zero, Cw, Cn, Cy, c1, s_m, nd, nb, ytn,
+                                        epsn, c1 >= epsn ? P.n : 0);
+        else
+            wave_box_rescan<false, LIKE>(zero, Cw, Cn, Cy, c1, s_m, nd, nb,
+                                         ytn, epsn, 0);
+        bi = wave_argmax(zero, mx);
+    }
+    if (lane == 0) {
+        if (bi == INT_MAX) {  // no window qualifies: power -inf as astropy
+            o.power[p] = -INFINITY;
+            o.depth[p] = 0.0;
+            o.depth_err[p] = 0.0;
+            o.duration[p] = 0.0;
+            o.transit_time[p] = P.t_ref;
+            o.depth_snr[p] = 0.0;
+            o.log_like[p] = 0.0;
+        } else {
+            const int k = bi >> 16, st = bi & 0xffff, m = s_m[k];
+            o.depth[p] = ((packed || counted)
+                              ? (double)(Cn[st + m] - Cn[st]) * c1
+                              : Cw[st + m] - Cw[st]) *
+                         Wt;
+            o.depth_err[p] = (Cy[st + m] - Cy[st]) * Wt;
+            o.duration[p] = -(double)bi - 1.0;
+        }
+    }
 }
 
 // Destinations of a point in raw bin r once the padding assignment
@@ -2168,6 +2379,14 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
             int *cntb = (int *)(binW + cap2);
             // the pass's slot s is the item's period l0 + s (lane l0 + s)
             const int l0 = (int)(p - pbeg);
+            // a pass whose first period leaves room for four of its size,
+            // each with the room for its own point counts (packed), takes
+            // periods so, and its waves finish them side by side
+            // (finish_period_wave); longer periods take the workgroup's
+            // finish, one at a time
+            const int nb0 = __builtin_amdgcn_readlane(nb_l, l0);
+            const bool wmode =
+                NW * (nb0 + 1 + (packed ? (nb0 + 2) / 2 : 0)) <= cap;
             double per[SUBMAX], inv[SUBMAX];
             int nb[SUBMAX], base[SUBMAX];
             int cnt = 0, used = 0;
@@ -2186,14 +2405,20 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
                     // sentinels: the overhang after its point counts
                     // (packed) or weights (unpacked)
                     const int ov = scan_overhang(nbs, mmax);
-                    if (used + (nbs + 1) +
-                            (packed ? max((nbs + 1 + ov + 1) / 2, ov) : ov) <=
-                        cap) {
+                    const int foot =
+                        wmode ? nbs + 1 + (packed ? (nbs + 2) / 2 : 0)
+                              : nbs + 1;
+                    const int need =
+                        wmode ? foot
+                              : nbs + 1 + (packed ? max((nbs + 1 + ov + 1) / 2,
+                                                        ov)
+                                                  : ov);
+                    if (used + need <= cap) {
                         per[s] = readlane_d(per_l, l0 + s);
                         inv[s] = readlane_d(inv_l, l0 + s);
                         nb[s] = nbs;
                         base[s] = used;
-                        used += nbs + 1;
+                        used += foot;
                         ++cnt;
                     }
                 }
@@ -2251,7 +2476,25 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
             }
             __syncthreads();
 
-            if (packed) {
+            if (wmode) {
+                // wave w finishes the pass's periods w, w + 4, ...
+                const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+                int bse = 0;
+                for (int s = 0; s < cnt; ++s) {
+                    const int nbs = __builtin_amdgcn_readlane(nb_l, l0 + s);
+                    if ((s & (NW - 1)) == wv) {
+                        if (packed)
+                            finish_period_wave<LIKE>(b, o, P, p + s, nbs, bse,
+                                                     binW, binW, nullptr, s_m,
+                                                     nd);
+                        else
+                            finish_period_wave<LIKE>(
+                                b, o, P, p + s, nbs, bse, binW, binY,
+                                counted ? cntb : nullptr, s_m, nd);
+                    }
+                    bse += nbs + 1 + (packed ? (nbs + 2) / 2 : 0);
+                }
+            } else if (packed) {
                 int end = used;
                 for (int s = cnt - 1; s >= 0; --s) {
                     const int nbs = __builtin_amdgcn_readlane(nb_l, l0 + s);
