@@ -1793,31 +1793,34 @@ __device__ __forceinline__ void pad_targets(int r, int nb, int os, int &j1,
     j2 = (r >= 1 && r <= os) ? r + nb - os - 1 : -1;
 }
 
-// One point into an interleaved bin: weight at bin[0], weight*flux at
-// bin[1] (one address, two atomics at offsets 0 and 8).
-__device__ __forceinline__ void bin_add(long long *bin, long long wv,
-                                        long long yv) {
-    __hip_atomic_fetch_add(bin, wv, __ATOMIC_RELAXED,
+// One point into bin l of a tile: weight (or count) at bw[l], weight*flux
+// at by[l].  Two arrays of 8-byte bins rather than interleaved pairs: the
+// lanes of a wave hold points ~1.4 bins apart, which an 8-byte stride
+// spreads over 32 of the 64 LDS bank pairs, a 16-byte one over 16.
+__device__ __forceinline__ void bin_add(long long *bw, long long *by, int l,
+                                        long long wv, long long yv) {
+    __hip_atomic_fetch_add(bw + l, wv, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_WORKGROUP);
-    __hip_atomic_fetch_add(bin + 1, yv, __ATOMIC_RELAXED,
+    __hip_atomic_fetch_add(by + l, yv, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 __device__ __forceinline__ void seg_add(int j, int s0, int hi, long long wv,
                                         long long yv, long long &cw,
-                                        long long &cy, long long *bins) {
+                                        long long &cy, long long *bw,
+                                        long long *by) {
     if (j < 0) return;
     if (j < s0) {
         cw += wv;
         cy += yv;
     } else if (j <= hi) {
-        bin_add(bins + 2 * (j - s0), wv, yv);
+        bin_add(bw, by, j - s0, wv, yv);
     }
 }
 
 // Ranged fold of one cycle's point range [a, e) of a time-sorted curve
-// into the LDS segment of bins [s0, hi] (interleaved: bin s0 + l at
-// bins[2l], bins[2l + 1]), one block of the point-order copy per step:
+// into the LDS segment of bins [s0, hi] (bin s0 + l at bw[l] and by[l]),
+// one block of the point-order copy per step:
 // thread tid takes slot tid, which holds point nat_slot^-1(tid) of the
 // block.  The range comes from lower_key with the fold of this mode, so
 // every point in it has its bin r (the oracle's in parity mode) in
@@ -1836,7 +1839,7 @@ template <bool FAST, bool PAD, bool UW>
 __device__ __forceinline__ void big_fold_range(
     const double *tn, const long long *wn, int a, int e,
     double period, double qc, double bd, double inv_bd, double tol_x, int nb,
-    int os, int s0, int hi, long long *bins, long long wone) {
+    int os, int s0, int hi, long long *bw, long long *by, long long wone) {
     if (a >= e) return;
     const unsigned tid = threadIdx.x;
     const int pt = (tid & 63) * NW + (tid >> 6);  // point of slot tid
@@ -1881,7 +1884,7 @@ __device__ __forceinline__ void big_fold_range(
         const unsigned l = (unsigned)(r + dsh);
         const bool in = (!masked || (unsigned)(B + pt - a) < cnt) &&
                         (!PAD || l <= span);
-        if (in) bin_add(bins + 2 * min(l, span), wv, yv);
+        if (in) bin_add(bw, by, min(l, span), wv, yv);
     };
     const int Bf = (a + NT - 1) & ~(NT - 1);  // first whole block
     const int Be = e & ~(NT - 1);             // end of the whole blocks
@@ -1909,7 +1912,7 @@ __device__ __forceinline__ void big_fold_range(
             if (fl) r = exact(t[u]);
             const unsigned l = (unsigned)(r + dsh);
             if (!PAD || l <= span)
-                bin_add(bins + 2 * min(l, span), wv[u], yv[u]);
+                bin_add(bw, by, min(l, span), wv[u], yv[u]);
         }
     }
     for (; B < Be; B += NT) fold_block(B, false);
@@ -1917,12 +1920,13 @@ __device__ __forceinline__ void big_fold_range(
 }
 
 // Box scan of one segment of a phase-tiled period: starts s0..nend-1 of
-// every duration over the segment's prefix values C2[l] = (weight, flux)
-// of bin s0 + l (uniform weights: the point count as weight, times c1), the
+// every duration over the segment's prefix values Cw[l] (weight; uniform
+// weights: the point count, times c1) and Cy[l] (flux) of bin s0 + l, the
 // thread taking starts s0 + tid, s0 + tid + NT, ...  sb receives the
 // thread's best window (the first of equal scores: its indices increase).
 template <bool PACKED, bool LIKE, bool CFORM>
-__device__ __forceinline__ void big_box_scan(Score &sb, const double2 *C2,
+__device__ __forceinline__ void big_box_scan(Score &sb, const double *Cw,
+                                             const double *Cy,
                                              int s0, int nend, int nb,
                                              const int *s_m, int nd,
                                              double c1, double ytn,
@@ -1939,10 +1943,9 @@ __device__ __forceinline__ void big_box_scan(Score &sb, const double2 *C2,
 #pragma unroll 4
         for (int n = s0 + (int)threadIdx.x; n <= last; n += NT) {
             const int l = n - s0;
-            const double2 c0 = C2[l], ce = C2[l + m];
-            const double yin = ce.y - c0.y;
+            const double yin = Cy[l + m] - Cy[l];
             if (CFORM) {
-                const double cnt = ce.x - c0.x;
+                const double cnt = Cw[l + m] - Cw[l];
                 const double wout = fma(-c1, cnt, 1.0);
                 const double num = fma(ytc, cnt, -yin);
                 sb.offer_later((cnt > 0.5) & (cnt < ntop) & !(num < 0.0),
@@ -1951,7 +1954,8 @@ __device__ __forceinline__ void big_box_scan(Score &sb, const double2 *C2,
                                k * nval + n);
                 continue;
             }
-            const double win = PACKED ? (ce.x - c0.x) * c1 : ce.x - c0.x;
+            const double win = PACKED ? (Cw[l + m] - Cw[l]) * c1
+                                      : Cw[l + m] - Cw[l];
             const double wout = 1.0 - win;
             const double num = fma(ytn, win, -yin);
             sb.offer_later(!(fmin(win, wout) < epsn) & !(num < 0.0),
@@ -1963,8 +1967,8 @@ __device__ __forceinline__ void big_box_scan(Score &sb, const double2 *C2,
 
 // One period whose n_bins + 1 exceeds the LDS histogram: the start
 // positions are processed in segments of S start positions; segment k keeps
-// bins [s0, s0 + S + mmax] in LDS (interleaved weight and weight*flux
-// sums, 16 B per bin) on top of the exact int64 sum of all bins below s0
+// bins [s0, s0 + S + mmax] in LDS (weight and weight*flux sums in two
+// arrays, 16 B per bin) on top of the exact int64 sum of all bins below s0
 // (the carried prefix).
 //
 // Ranged form (time-sorted curves): the points of cycle c whose bin lies in
@@ -2023,6 +2027,10 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
         }
     }
     int *tab = (int *)(bins + 2 * (lds_bins - R));
+    // the tile: weights (uniform weights: point counts) in bw, weight*flux
+    // sums in by, TB bins each, then the range table
+    const int TB = lds_bins - R;
+    long long *bw = bins, *by = bins + TB;
     int *erng = tab + 2 * ncyc * (batched ? nseg : 1);
     const long long *wn = w.nat_w + nbase;
     Score best;
@@ -2032,7 +2040,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
     const long long *yp = w.ws_y + P.ws_off + tid;
     // the prefix values, in place: C[2i] the weight (uniform weights: the
     // point count), C[2i + 1] the weight*flux sum
-    double *C = (double *)bins;
+    double *Cw = (double *)bw, *Cy = (double *)by;
     if (tid == 0) {
         s_carry[0] = 0;
         s_carry[1] = 0;
@@ -2057,7 +2065,10 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
             cw = s_carry[0];
             cy = s_carry[1];
         }
-        for (int i = tid; i < 2 * len; i += NT) bins[i] = 0;
+        for (int i = tid; i < len; i += NT) {
+            bw[i] = 0;
+            by[i] = 0;
+        }
         const bool extra = hi >= nb - os;  // padding copies land here
         if (ranged && !batched) {
             const int ntask = (extra ? 4 : 2) * ncyc;
@@ -2077,7 +2088,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
             // padding copies, bins 1..os of every cycle to nb-os..nb-1
 #define ATG_RANGE(F, PD, U, J0, E)                                           \
     big_fold_range<F, PD, U>(tn, wn, J0, E, period, (double)c, bd,           \
-                             inv_bd, P.tol_x, nb, os, s0, hi, bins, 1)
+                             inv_bd, P.tol_x, nb, os, s0, hi, bw, by, 1)
 #define ATG_RANGES(PD, TAB)                                                  \
     for (int c = 0; c < ncyc; ++c) {                                         \
         const int j0 = TAB[2 * c], e = TAB[2 * c + 1];                       \
@@ -2096,8 +2107,8 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
                 __syncthreads();
                 const int z0 = max(nb - os, s0), z1 = min(nb - 1, hi);
                 for (int i = z0 + tid; i <= z1; i += NT) {
-                    bins[2 * (i - s0)] = 0;
-                    bins[2 * (i - s0) + 1] = 0;
+                    bw[i - s0] = 0;
+                    by[i - s0] = 0;
                 }
                 __syncthreads();
                 ATG_RANGES(true, erng)
@@ -2119,8 +2130,8 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
                                 nb);
                 int j1, j2;
                 pad_targets(r, nb, os, j1, j2);
-                seg_add(j1, s0, hi, wv, yv, cw, cy, bins);
-                seg_add(j2, s0, hi, wv, yv, cw, cy, bins);
+                seg_add(j1, s0, hi, wv, yv, cw, cy, bw, by);
+                seg_add(j2, s0, hi, wv, yv, cw, cy, bw, by);
             }
             __syncthreads();
             cw = block_reduce(cw, AddOp<long long>(), lscr);
@@ -2128,41 +2139,36 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
         }
         const int seg = (len + NT - 1) / NT;
         const int b0 = min(tid * seg, len), b1 = min(b0 + seg, len);
-        const longlong2 *B2 = (const longlong2 *)bins;
         long long sw = 0, sy = 0;
         for (int i = b0; i < b1; ++i) {
-            const longlong2 v = B2[i];
-            sw += v.x;
-            sy += v.y;
+            sw += bw[i];
+            sy += by[i];
         }
         long long tw, ty;
         long long ow = block_excl_scan_dpp(sw, lscr, &tw) + cw;
         long long oy = block_excl_scan_dpp(sy, lscr, &ty) + cy;
         for (int i = b0; i < b1; ++i) {
-            const longlong2 v = B2[i];
-            ow += v.x;
-            oy += v.y;
+            ow += bw[i];
+            oy += by[i];
             if (i == S - 1) {  // prefix of the next segment's first bin
                 s_carry[0] = ow;
                 s_carry[1] = oy;
             }
             // uniform weights: the point count, as in finish_period (the
             // weight channel counts points)
-            double2 cv;
-            cv.x = P.uniform ? (double)ow : (double)ow * fw;
-            cv.y = (double)oy * fy;
-            ((double2 *)C)[i] = cv;
+            Cw[i] = P.uniform ? (double)ow : (double)ow * fw;
+            Cy[i] = (double)oy * fy;
         }
         __syncthreads();
         const int nend = min(s0 + S, nb);
-        const double2 *C2 = (const double2 *)C;
         // duration-major, as in finish_period: a thread meets the segment's
         // windows in increasing index, so the segment's best needs no tie
         // rule; it joins the period's best with the full one (indices are
         // not monotone across segments)
         Score sb;
 #define ATG_BBS(U, L, CF)                                                    \
-    big_box_scan<U, L, CF>(sb, C2, s0, nend, nb, s_m, nd, c1, ytn, epsn, P.n)
+    big_box_scan<U, L, CF>(sb, Cw, Cy, s0, nend, nb, s_m, nd, c1, ytn, epsn,  \
+                           P.n)
         if (P.uniform && c1 >= epsn) {
             if (like) ATG_BBS(true, true, true);
             else ATG_BBS(true, false, true);
@@ -2180,9 +2186,8 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
             if (best.idx != before) {  // the window's sums, read back
                 const int k = sb.idx / nval, n = sb.idx - k * nval;
                 const int l = n - s0, m = s_m[k];
-                const double2 c0 = C2[l], ce = C2[l + m];
-                bwin = P.uniform ? (ce.x - c0.x) * c1 : ce.x - c0.x;
-                byin = ce.y - c0.y;
+                bwin = P.uniform ? (Cw[l + m] - Cw[l]) * c1 : Cw[l + m] - Cw[l];
+                byin = Cy[l + m] - Cy[l];
             }
         }
         __syncthreads();
@@ -2533,7 +2538,7 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4)))
 void k_big(Batch b, Out o, Work w, int lds_bins) {
     extern __shared__ __align__(16) unsigned char smem[];
-    long long *bins = (long long *)smem;  // lds_bins interleaved bins
+    long long *bins = (long long *)smem;  // 2 * lds_bins int64 (the tile)
     __shared__ int s_m[DMAX];
     __shared__ unsigned int s_j;
     __shared__ long long lscr[NW];
