@@ -86,8 +86,6 @@ struct LcParams {
     int status;       // ATG_LC_*
     int n_dur;
     double tol_x;     // near-integer tolerance of phase/bd (uniform bound)
-    double seq_y;     // parity mode: the oracle's totals, summed in FP64
-    double seq_iv;    // in point order (sum of y*ivar, sum of ivar)
     int sorted;       // t non-decreasing (enables the ranged phase tiles)
     int lc_self;      // index of the curve
     int natural;      // natural-order copies of the points were written
@@ -121,6 +119,8 @@ struct Out {
 
 struct Work {
     LcParams *prm;
+    double2 *seq;         // [n_lc] parity mode: the oracle's totals, summed
+                          // in FP64 in point order (sum of y*ivar, of ivar)
     int *lc_flags;        // per-curve flags raised by k_main (LDS overflow)
     long long *item_off;  // [n_lc] first item of the curve in its queue
     long long *qtot;      // [NQ] items per queue
@@ -158,6 +158,41 @@ __device__ __forceinline__ double load_val(const void *p, int dt,
                                            long long i) {
     return dt == ATG_F32 ? (double)((const float *)p)[i]
                          : ((const double *)p)[i];
+}
+
+// The raw inputs with their types fixed at compile time: y FP32 or FP64
+// (YT), dy absent (DT 0), FP32 or FP64.  The kernels that stream a curve's
+// raw points (k_prep, k_exact) are instantiated per type pair, so that their
+// loads carry no branch: a branch around each load makes the compiler wait
+// for every load before the next, which serialises the memory latency.
+template <int YT, int DT>
+struct Raw {
+    static constexpr int Y_T = YT;
+    static constexpr bool HAS_DY = DT != 0;
+    static __device__ __forceinline__ double y(const Batch &b, long long i) {
+        if (YT == ATG_F32) return (double)((const float *)b.y)[i];
+        return ((const double *)b.y)[i];
+    }
+    static __device__ __forceinline__ double dy(const Batch &b, long long i) {
+        if (DT == ATG_F32) return (double)((const float *)b.dy)[i];
+        if (DT == ATG_F64) return ((const double *)b.dy)[i];
+        return 1.0;
+    }
+};
+
+// Calls f(Raw<y type, dy type>()) for the batch's input types (host side).
+template <class F>
+void with_raw_types(const Batch &b, F f) {
+    const int dt = b.dy ? b.dy_dt : 0;
+    if (b.y_dt == ATG_F32) {
+        if (dt == 0) f(Raw<ATG_F32, 0>());
+        else if (dt == ATG_F32) f(Raw<ATG_F32, ATG_F32>());
+        else f(Raw<ATG_F32, ATG_F64>());
+    } else {
+        if (dt == 0) f(Raw<ATG_F64, 0>());
+        else if (dt == ATG_F32) f(Raw<ATG_F64, ATG_F32>());
+        else f(Raw<ATG_F64, ATG_F64>());
+    }
 }
 
 // Value of lane j (uniform j) of an FP64 register, as a wave-uniform value.
@@ -596,27 +631,33 @@ __device__ __forceinline__ void bin_points(
 
 // ------------------------------------------------------------------ k_prep
 
-// Two order statistics k0 <= k1 of the curve's y values (as FP64) by an
-// 8-pass radix select over the order-preserving 64-bit key.
-__device__ void select2(const void *y, int dt, long long off, int n, int k0,
-                        int k1, double *v0, double *v1, unsigned int *hist,
-                        int *sel, unsigned int *uscratch) {
+// Two order statistics k0 <= k1 of the curve's y values (as FP64) by a
+// radix select over the order-preserving 64-bit key, 8 bits a pass.  FP32
+// values widened to FP64 have their 29 lowest mantissa bits zero, so their
+// keys' lowest 24 bits are all zeros (positive values) or all ones
+// (negative: the key inverts every bit); five passes fix the other 40 bits
+// and the sign of the prefix fixes the rest.
+template <class R>
+__device__ void select2(const Batch &b, long long off, int n, int k0, int k1,
+                        double *v0, double *v1, unsigned int *hist, int *sel,
+                        unsigned int *uscratch) {
+    constexpr int PASSES = R::Y_T == ATG_F32 ? 5 : 8;
     unsigned long long pre0 = 0, pre1 = 0, mask = 0;
     int kk0 = k0, kk1 = k1;
-    for (int pass = 0; pass < 8; ++pass) {
+    for (int pass = 0; pass < PASSES; ++pass) {
         const int shift = 56 - 8 * pass;
         for (int i = threadIdx.x; i < 512; i += NT) hist[i] = 0;
         __syncthreads();
         unsigned c0 = 0, c1 = 0, d0 = 0, d1 = 0;
-        // four values per step, loaded before the first is counted (the
+        // eight values per step, loaded before the first is counted (the
         // thread's order of points is unchanged)
-        for (int i0 = threadIdx.x; i0 < n; i0 += 4 * NT) {
-            double v[4];
+        for (int i0 = threadIdx.x; i0 < n; i0 += 8 * NT) {
+            double v[8];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                v[u] = load_val(y, dt, off + min(i0 + u * NT, n - 1));
+            for (int u = 0; u < 8; ++u)
+                v[u] = R::y(b, off + min(i0 + u * NT, n - 1));
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < 8; ++u) {
                 if (i0 + u * NT >= n) break;
                 unsigned long long key = okey(v[u]);
                 unsigned d = (unsigned)(key >> shift) & 255u;
@@ -657,42 +698,41 @@ __device__ void select2(const void *y, int dt, long long off, int n, int k0,
         mask |= 0xffull << shift;
         __syncthreads();
     }
+    if (PASSES == 5) {
+        if (!(pre0 >> 63)) pre0 |= 0xffffffull;
+        if (!(pre1 >> 63)) pre1 |= 0xffffffull;
+    }
     *v0 = ikey(pre0);
     *v1 = ikey(pre1);
 }
 
 // Raw y and dy of point i (dy = 1 when the curve has none).
+template <class R>
 __device__ __forceinline__ void load_point(const Batch &b, long long i,
                                            double &y, double &d) {
-    y = load_val(b.y, b.y_dt, i);
-    d = b.dy ? load_val(b.dy, b.dy_dt, i) : 1.0;
+    y = R::y(b, i);
+    d = R::dy(b, i);
 }
 
 // Fixed-point weight and weight*flux of a point from its raw y and dy.
-__device__ __forceinline__ void quantize_raw(const Batch &b, double y,
-                                             double d, double med,
+template <class R>
+__device__ __forceinline__ void quantize_raw(double y, double d, double med,
                                              double wscale, double yscale,
                                              long long &wq, long long &yq) {
     double iv = 1.0;
-    if (b.dy) iv = 1.0 / (d * d);
+    if (R::HAS_DY) iv = 1.0 / (d * d);
     const double yc = y - med;
     wq = llrint(iv * wscale);
     yq = llrint((yc * iv) * yscale);
 }
 
-// Fixed-point weight and weight*flux of point i (shared by both layouts).
-__device__ __forceinline__ void quantize(const Batch &b, long long i,
-                                         double med, double wscale,
-                                         double yscale, long long &wq,
-                                         long long &yq) {
-    double y, d;
-    load_point(b, i, y, d);
-    quantize_raw(b, y, d, med, wscale, yscale, wq, yq);
-}
-
+template <class R>
 __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     __shared__ unsigned int hist[512];
-    __shared__ double seq_v[8 * NT];
+    // the sequential totals' chunks: two buffers of SEQ_CH (term, weight)
+    // pairs, each padded by 8 for the summing wave's read-ahead
+    constexpr int SEQ_CH = 3 * 64 * 4;
+    __shared__ double2 seq_v[2 * (SEQ_CH + 8)];
     __shared__ int sel[4];
     __shared__ unsigned int uscr[NW];
     __shared__ double dscr[NW];
@@ -770,7 +810,7 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
             const int i = min(i0 + u * NT, n - 1);
             tv[u] = b.t[off + i];
             tp[u] = b.t[off + max(i - 1, 0)];
-            load_point(b, off + i, yv[u], dv[u]);
+            load_point<R>(b, off + i, yv[u], dv[u]);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -779,7 +819,7 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
             tmin = fmin(tmin, tv[u]);
             if (isnan(tv[u]) || isnan(yv[u])) bad |= 1;
             if (i > 0 && !(tv[u] >= tp[u])) bad |= 4;
-            if (b.dy && !(dv[u] > 0.0)) bad |= 2;
+            if (R::HAS_DY && !(dv[u] > 0.0)) bad |= 2;
         }
     }
     tmin = block_reduce(tmin, MinOp(), dscr);
@@ -799,43 +839,86 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
 
     // exact median (numpy: mean of the two middle values for even n)
     double v0, v1;
-    select2(b.y, b.y_dt, off, n, (n - 1) / 2, n / 2, &v0, &v1, hist, sel,
-            uscr);
+    select2<R>(b, off, n, (n - 1) / 2, n / 2, &v0, &v1, hist, sel, uscr);
     const double med = (n & 1) ? v0 : (v0 + v1) * 0.5;
     P.y_med = med;
 
-    // parity mode: the oracle's FP64 totals in its order (one wave walks
-    // the points; the refinement kernel uses them)
-    // The workgroup stages chunks of 4*NT values in LDS (four loads per
-    // thread in flight) and the first wave adds them up in point order from
-    // broadcast reads.
+    // parity mode: the oracle's FP64 totals in its order (the refinement
+    // kernel uses them).  Each total is one chain of dependent FP64 adds
+    // (~20 cycles each), so one wave adds while the others stage: waves 1-3
+    // put the next chunk of SEQ_CH (term, weight) pairs in LDS while wave 0
+    // adds up the current one in point order from broadcast reads, eight
+    // pairs read ahead of their adds.
     if (b.mode == ATG_MODE_PARITY) {
-        double sy = 0.0, si = 0.0;
-        for (int base = 0; base < n; base += 4 * NT) {
+        constexpr int SPAN = 3 * 64;  // staging threads
+        const int nch = (n + SEQ_CH - 1) / SEQ_CH;
+        auto stage = [&](int c) {
+            double2 *q = seq_v + (c & 1) * (SEQ_CH + 8);
+            const int j = tid - 64, c0 = c * SEQ_CH;
             double yr[4], dr[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                load_point(b, off + min(base + u * NT + tid, n - 1), yr[u],
-                           dr[u]);
+                load_point<R>(b, off + min(c0 + u * SPAN + j, n - 1), yr[u],
+                              dr[u]);
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const double iv = b.dy ? 1.0 / (dr[u] * dr[u]) : 1.0;
-                seq_v[u * NT + tid] = (yr[u] - med) * iv;
-                seq_v[4 * NT + u * NT + tid] = iv;
+                const double iv = R::HAS_DY ? 1.0 / (dr[u] * dr[u]) : 1.0;
+                q[u * SPAN + j] = make_double2((yr[u] - med) * iv, iv);
             }
-            __syncthreads();
-            if (tid < 64) {
-                const int cnt = min(4 * NT, n - base);
-#pragma unroll 8
-                for (int k = 0; k < cnt; ++k) {
-                    sy = sy + seq_v[k];
-                    si = si + seq_v[4 * NT + k];
+        };
+        double sy = 0.0, si = 0.0;
+        if (tid >= 64) stage(0);
+        __syncthreads();
+        for (int c = 0; c < nch; ++c) {
+            if (tid >= 64) {
+                if (c + 1 < nch) stage(c + 1);
+            } else {
+                const double2 *q = seq_v + (c & 1) * (SEQ_CH + 8);
+                const int cnt = min(SEQ_CH, n - c * SEQ_CH);
+                const int full = cnt & ~7;
+                // reads reach index full + 7 at most: within the padding;
+                // the scheduling barriers keep each group's reads ahead of
+                // the other group's adds
+                double2 A[8], B[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) A[u] = q[u];
+                int k = 0;
+                for (; k + 16 <= full; k += 16) {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) B[u] = q[k + 8 + u];
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        sy = sy + A[u].x;
+                        si = si + A[u].y;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) A[u] = q[k + 16 + u];
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        sy = sy + B[u].x;
+                        si = si + B[u].y;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                if (k + 8 <= full) {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        sy = sy + A[u].x;
+                        si = si + A[u].y;
+                    }
+                    k += 8;
+                }
+                for (; k < cnt; ++k) {
+                    sy = sy + q[k].x;
+                    si = si + q[k].y;
                 }
             }
             __syncthreads();
         }
-        P.seq_y = sy;
-        P.seq_iv = si;
+        if (tid == 0) w.seq[lc] = make_double2(sy, si);
     }
 
     // pass B: magnitudes for the fixed-point scales, weight uniformity, and
@@ -847,12 +930,12 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
         double yr[4], dr[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            load_point(b, off + min(i0 + u * NT, n - 1), yr[u], dr[u]);
+            load_point<R>(b, off + min(i0 + u * NT, n - 1), yr[u], dr[u]);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (i0 + u * NT >= n) break;
             double iv = 1.0;
-            if (b.dy) iv = 1.0 / (dr[u] * dr[u]);
+            if (R::HAS_DY) iv = 1.0 / (dr[u] * dr[u]);
             double yc = yr[u] - med;
             const double a = fabs(yc * iv);
             sw_sum += iv;
@@ -1016,7 +1099,7 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
         for (int u = 0; u < 4; ++u) {
             const int src = min(i0 + u * 64 + ln, n - 1);
             tr[u] = b.t[off + src];
-            load_point(b, off + src, yr[u], dr[u]);
+            load_point<R>(b, off + src, yr[u], dr[u]);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -1029,7 +1112,7 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
             carry += __popcll(m);
             if (!valid) continue;
             long long wq, yq;
-            quantize_raw(b, yr[u], dr[u], med, wscale, yscale, wq, yq);
+            quantize_raw<R>(yr[u], dr[u], med, wscale, yscale, wq, yq);
             long long slot;
             if (edge) {
                 slot = P.ws_off + (long long)(L0 + e / NT) * NT + e % NT;
@@ -1080,7 +1163,7 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
             for (int u = 0; u < 4; ++u) {
                 const int i = min(i0 + u * NT, n - 1);
                 tr[u] = b.t[off + i];
-                load_point(b, off + i, yr[u], dr[u]);
+                load_point<R>(b, off + i, yr[u], dr[u]);
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
@@ -1089,7 +1172,7 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
                 long long wq = 0, yq = 0;
                 double tv = 0.0;
                 if (i < n) {
-                    quantize_raw(b, yr[u], dr[u], med, wscale, yscale, wq,
+                    quantize_raw<R>(yr[u], dr[u], med, wscale, yscale, wq,
                                  yq);
                     tv = tr[u] - tmin;
                 }
@@ -2726,15 +2809,26 @@ __global__ __launch_bounds__(NT) void k_cands(Batch b, Out o, Work w) {
     }
 }
 
+// A raw point of k_exact's walk (range index k for the ranged form), and how
+// many chunks of 64 points ahead of the current one the walk loads.
+struct ExactPt {
+    double t = 0.0, y = 0.0, d = 1.0;
+    int k = 0;
+};
+constexpr int EXACT_AHEAD = 4;
+
 // One wave's chunk of points added into the LDS bins my/mi so that every
 // bin receives its points in chunk order (the oracle's point order): lane j
 // (j < cnt) adds (yv, iv) to bin dst (-1: to none).  (cur, ay, ai) is the
 // bin whose running sums the item-by-item walk keeps in registers (cur -1:
-// none).  When the chunk's destinations never decrease (time-sorted points
-// within one cycle), each bin's points are one run of lanes, preceded only
-// by the points that go nowhere: the first lane of every run adds the run's
-// values one after another from LDS, all runs at once.  Any other chunk is
-// walked point by point.  The sums are the same either way, add for add.
+// none).  The chunk is cut before every lane whose destination is below its
+// predecessor's (a cycle wrap of time-sorted points, or a point that goes
+// nowhere after others): within each piece the destinations never decrease,
+// so each bin's points are one run of lanes, preceded only by the points
+// that go nowhere, and the first lane of every run adds the run's values one
+// after another from LDS, all runs of the piece at once; the pieces follow
+// one another.  A chunk of more than four pieces is walked point by point.
+// The sums are the same either way, add for add.
 __device__ __forceinline__ void exact_add_chunk(int dst, double yv, double iv,
                                                 int cnt, double *my,
                                                 double *mi, double *st_y,
@@ -2744,7 +2838,8 @@ __device__ __forceinline__ void exact_add_chunk(int dst, double yv, double iv,
     const bool act = lane < cnt;
     const int prev = __shfl_up(dst, 1);
     const bool down = act && lane > 0 && dst < prev;
-    if (__ballot(down) == 0) {
+    unsigned long long cuts = __ballot(down);
+    if (__popcll(cuts) <= 3) {
         if (cur >= 0) {
             if (lane == 0) {
                 my[cur] = ay;
@@ -2755,21 +2850,31 @@ __device__ __forceinline__ void exact_add_chunk(int dst, double yv, double iv,
         st_y[lane] = yv;
         st_i[lane] = iv;
         __builtin_amdgcn_wave_barrier();
-        const unsigned long long starts =
-            __ballot(act && (lane == 0 || dst != prev));
-        if (act && dst >= 0 && (lane == 0 || dst != prev)) {
-            const unsigned long long later =
-                lane < 63 ? starts >> (lane + 1) : 0ull;
-            const int end = later ? lane + 1 + __builtin_ctzll(later) : cnt;
-            double a = my[dst], c = mi[dst];
-            for (int k = lane; k < end; ++k) {
-                a += st_y[k];
-                c += st_i[k];
+        int lo = 0;
+        for (;;) {
+            const int hi = cuts ? __builtin_ctzll(cuts) : cnt;
+            const bool in = lane >= lo && lane < hi;
+            const bool start = in && (lane == lo || dst != prev);
+            const unsigned long long starts = __ballot(start);
+            if (start && dst >= 0) {
+                const unsigned long long later =
+                    lane < 63 ? starts >> (lane + 1) : 0ull;
+                const int end = later ? lane + 1 + __builtin_ctzll(later) : hi;
+                double a = my[dst], c = mi[dst];
+                for (int k = lane; k < end; ++k) {
+                    a += st_y[k];
+                    c += st_i[k];
+                }
+                my[dst] = a;
+                mi[dst] = c;
             }
-            my[dst] = a;
-            mi[dst] = c;
+            // the piece's sums are stored before the next piece reads them
+            // (one wave: its LDS accesses complete in order)
+            __builtin_amdgcn_wave_barrier();
+            if (!cuts) break;
+            lo = hi;
+            cuts &= cuts - 1;
         }
-        __builtin_amdgcn_wave_barrier();
         return;
     }
     for (int j = 0; j < cnt; ++j) {
@@ -2791,6 +2896,7 @@ __device__ __forceinline__ void exact_add_chunk(int dst, double yv, double iv,
     }
 }
 
+template <class R>
 __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
                                               int lds_bins) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -2855,8 +2961,8 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
     double bo = -INFINITY, byin = 0.0, bivin = 0.0;
     int bi = INT_MAX;
     if (lane == 0) {
-        s_tot[0] = P.seq_y;
-        s_tot[1] = P.seq_iv;
+        s_tot[0] = w.seq[lc].x;
+        s_tot[1] = w.seq[lc].y;
     }
     for (int s0 = 0; s0 < nb; s0 += S) {
         const int hi = fits ? nb : min(s0 + S + mmax, nb);
@@ -2893,73 +2999,77 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
             }
             __syncthreads();
             const int total = pre_t[2 * ncyc];
-            // item -> (range, point); the next chunk loads while the wave
-            // walks the current one
-            int nk = 0, nj = -1;
-            double t_raw = 0.0, y_raw = 0.0, d_raw = 1.0;
-            auto locate = [&](int item, int &k, int &j) {
-                int a = 0, z = 2 * ncyc;  // last k with pre_t[k] <= item
-                while (z - a > 1) {
-                    const int mid = (a + z) >> 1;
-                    if (pre_t[mid] <= item) a = mid;
-                    else z = mid;
+            // item -> (range, point): a lane's items rise by 64 from one
+            // chunk to the next, so its range cursor only moves forward (one
+            // LDS read a chunk, more where ranges end).  The raw points of
+            // the next EXACT_AHEAD chunks are in flight while the wave walks
+            // the current one.
+            int kc = 0;
+            auto fetch = [&](int item, ExactPt &r) {
+                if (item < total) {
+                    while (pre_t[kc + 1] <= item) ++kc;
+                    const int j = lo_t[kc] + (item - pre_t[kc]);
+                    r.k = kc;
+                    r.t = b.t[off + j];
+                    r.y = R::y(b, off + j);
+                    r.d = R::dy(b, off + j);
                 }
-                k = a;
-                j = lo_t[a] + (item - pre_t[a]);
             };
-            if (lane < total) {
-                locate(lane, nk, nj);
-                t_raw = b.t[off + nj];
-                y_raw = load_val(b.y, b.y_dt, off + nj);
-                if (b.dy) d_raw = load_val(b.dy, b.dy_dt, off + nj);
-            }
-            for (int cb = 0; cb < total; cb += 64) {
-                const int k = nk;
-                const bool valid = cb + lane < total;
-                const double tp = t_raw - P.t_ref;
-                const double iv = b.dy ? 1.0 / (d_raw * d_raw) : 1.0;
-                const double yv = (y_raw - P.y_med) * iv;
-                int loc = -1;
-                if (valid) {
-                    const int c = k < ncyc ? k : k - ncyc;
-                    const int r = fold_bin_in(tp, period, (double)c, bd,
-                                              inv_bd, nb);
-                    int dst;
-                    if (k < ncyc) dst = (r < nb - os || r == nb) ? r : -1;
-                    else dst = (r >= 1 && r <= os) ? r + nb - os - 1 : -1;
-                    if (dst >= s0 && dst <= hi) loc = dst - s0;
+            ExactPt q[EXACT_AHEAD];
+#pragma unroll
+            for (int u = 0; u < EXACT_AHEAD; ++u) fetch(64 * u + lane, q[u]);
+            for (int cb0 = 0; cb0 < total; cb0 += 64 * EXACT_AHEAD) {
+#pragma unroll
+                for (int u = 0; u < EXACT_AHEAD; ++u) {
+                    const int cb = cb0 + 64 * u;
+                    if (cb >= total) break;
+                    const ExactPt r = q[u];
+                    fetch(cb + 64 * EXACT_AHEAD + lane, q[u]);
+                    const int k = r.k;
+                    const bool valid = cb + lane < total;
+                    const double tp = r.t - P.t_ref;
+                    const double iv = R::HAS_DY ? 1.0 / (r.d * r.d) : 1.0;
+                    const double yv = (r.y - P.y_med) * iv;
+                    int loc = -1;
+                    if (valid) {
+                        const int c = k < ncyc ? k : k - ncyc;
+                        const int rb = fold_bin_in(tp, period, (double)c, bd,
+                                                   inv_bd, nb);
+                        int dst;
+                        if (k < ncyc) dst = (rb < nb - os || rb == nb) ? rb : -1;
+                        else dst = (rb >= 1 && rb <= os) ? rb + nb - os - 1 : -1;
+                        if (dst >= s0 && dst <= hi) loc = dst - s0;
+                    }
+                    exact_add_chunk(loc, yv, iv, min(64, total - cb), my, mi,
+                                    st_y, st_i, cur, ay, ai);
                 }
-                if (cb + 64 + lane < total) {
-                    locate(cb + 64 + lane, nk, nj);
-                    t_raw = b.t[off + nj];
-                    y_raw = load_val(b.y, b.y_dt, off + nj);
-                    if (b.dy) d_raw = load_val(b.dy, b.dy_dt, off + nj);
-                }
-                exact_add_chunk(loc, yv, iv, min(64, total - cb), my, mi,
-                                st_y, st_i, cur, ay, ai);
             }
         } else {
         __syncthreads();
-        // raw inputs of the first chunk; each later chunk is loaded while
+        // the raw points of the next EXACT_AHEAD chunks are in flight while
         // the wave walks the current one
-        double t_raw = 0.0, y_raw = 0.0, d_raw = 1.0;
-        if (lane < n) {
-            t_raw = b.t[off + lane];
-            y_raw = load_val(b.y, b.y_dt, off + lane);
-            if (b.dy) d_raw = load_val(b.dy, b.dy_dt, off + lane);
-        }
-        for (int base = 0; base < n; base += 64) {
-            const double tp = t_raw - P.t_ref;
-            const double iv = b.dy ? 1.0 / (d_raw * d_raw) : 1.0;
-            const double yv = (y_raw - P.y_med) * iv;
+        auto fetch = [&](int i, ExactPt &r) {
+            if (i < n) {
+                r.t = b.t[off + i];
+                r.y = R::y(b, off + i);
+                r.d = R::dy(b, off + i);
+            }
+        };
+        ExactPt q[EXACT_AHEAD];
+#pragma unroll
+        for (int u = 0; u < EXACT_AHEAD; ++u) fetch(64 * u + lane, q[u]);
+        for (int base0 = 0; base0 < n; base0 += 64 * EXACT_AHEAD) {
+#pragma unroll
+        for (int u = 0; u < EXACT_AHEAD; ++u) {
+            const int base = base0 + 64 * u;
+            if (base >= n) break;
+            const ExactPt r = q[u];
+            fetch(base + 64 * EXACT_AHEAD + lane, q[u]);
+            const double tp = r.t - P.t_ref;
+            const double iv = R::HAS_DY ? 1.0 / (r.d * r.d) : 1.0;
+            const double yv = (r.y - P.y_med) * iv;
             const int ind = min(max(fold_bin(tp, period, invP, bd, inv_bd), 0),
                                 nb);
-            const int nx = base + 64 + lane;
-            if (nx < n) {
-                t_raw = b.t[off + nx];
-                y_raw = load_val(b.y, b.y_dt, off + nx);
-                if (b.dy) d_raw = load_val(b.dy, b.dy_dt, off + nx);
-            }
             const int cnt = min(64, n - base);
             if (fits) {
                 exact_add_chunk(ind, yv, iv, cnt, my, mi, st_y, st_i, cur,
@@ -2990,6 +3100,7 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
                     }
                 }
             }
+        }
         }
         }
         __syncthreads();
@@ -3762,7 +3873,7 @@ int hip_fail(atg_ctx *c, hipError_t e, const char *what) {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct Layout {
-    size_t prm, flags, items, queues, counter, cand_lc, cand_p, t, w, y,
+    size_t prm, seq, flags, items, queues, counter, cand_lc, cand_p, t, w, y,
         nt, nw, tk_pos, tk_st, big_q, big_p, big_lc, total;
 };
 
@@ -3772,6 +3883,7 @@ Layout layout_of(const atg_bls_batch *b) {
     const size_t slots = (size_t)b->n_points + nlc * (2 * NT);
     size_t o = 0;
     L.prm = o;     o = align256(o + nlc * sizeof(LcParams));
+    L.seq = o;     o = align256(o + nlc * 2 * sizeof(double));
     L.flags = o;   o = align256(o + nlc * sizeof(int));
     L.items = o;   o = align256(o + (nlc + 1) * sizeof(long long));
     L.queues = o;  o = align256(o + NQ * (sizeof(long long) + sizeof(int)));
@@ -3901,6 +4013,7 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
     const bool timing = c->timing && own_ws;
     Work w;
     w.prm = (LcParams *)(base + L.prm);
+    w.seq = (double2 *)(base + L.seq);
     w.lc_flags = (int *)(base + L.flags);
     w.item_off = (long long *)(base + L.items);
     w.counter = (unsigned int *)(base + L.counter);
@@ -3982,8 +4095,10 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
         }
         c->lds_attr_set = lds;
     }
-    hipLaunchKernelGGL(k_prep, dim3(nlc), dim3(NT), 0, c->stream, kb, w,
-                       c->tune.lds_bins);
+    with_raw_types(kb, [&](auto r) {
+        hipLaunchKernelGGL(k_prep<decltype(r)>, dim3(nlc), dim3(NT), 0,
+                           c->stream, kb, w, c->tune.lds_bins);
+    });
     hipLaunchKernelGGL(k_items, dim3(1), dim3(NT), 0, c->stream,
                        (long long)b->n_lc, kb.per_off, per_item, w);
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -4025,8 +4140,11 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
     }
     hipLaunchKernelGGL(k_cands, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w);
     const int ex_bins = std::min(c->tune.lds_bins, EXACT_BINS);
-    hipLaunchKernelGGL(k_exact, dim3(c->num_cu * EXACT_WG_PER_CU), dim3(64),
-                       (size_t)ex_bins * 16, c->stream, kb, ko, w, ex_bins);
+    with_raw_types(kb, [&](auto r) {
+        hipLaunchKernelGGL(k_exact<decltype(r)>,
+                           dim3(c->num_cu * EXACT_WG_PER_CU), dim3(64),
+                           (size_t)ex_bins * 16, c->stream, kb, ko, w, ex_bins);
+    });
     hipLaunchKernelGGL(k_final, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w,
                        kbest);
     e = hipGetLastError();
