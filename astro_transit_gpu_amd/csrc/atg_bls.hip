@@ -2780,6 +2780,12 @@ __global__ __launch_bounds__(NT) void k_cands(Batch b, Out o, Work w) {
     }
 }
 
+// A compile-time flag passed as a value (to a generic lambda).
+template <bool B>
+struct BoolC {
+    static constexpr bool value = B;
+};
+
 // A raw point of k_exact's walk (range index k for the ranged form), and how
 // many chunks of 64 points ahead of the current one the walk loads.
 struct ExactPt {
@@ -2796,10 +2802,13 @@ constexpr int EXACT_AHEAD = 4;
 // predecessor's (a cycle wrap of time-sorted points, or a point that goes
 // nowhere after others): within each piece the destinations never decrease,
 // so each bin's points are one run of lanes, preceded only by the points
-// that go nowhere, and the first lane of every run adds the run's values one
-// after another from LDS, all runs of the piece at once; the pieces follow
-// one another.  A chunk of more than four pieces is walked point by point.
-// The sums are the same either way, add for add.
+// that go nowhere.  The chunk's values are staged in LDS and the first lane
+// of every run adds its run's values to the bin's sums one after another,
+// all runs of a piece at once; the values are read eight at a time ahead of
+// their adds (a dense curve puts dozens of points in a bin per cycle, and
+// one LDS round trip per add would dominate).  The pieces follow one
+// another.  A chunk of more than four pieces is walked point by point.  The
+// sums are the same either way, add for add.
 __device__ __forceinline__ void exact_add_chunk(int dst, double yv, double iv,
                                                 int cnt, double *my,
                                                 double *mi, double *st_y,
@@ -2832,9 +2841,21 @@ __device__ __forceinline__ void exact_add_chunk(int dst, double yv, double iv,
                     lane < 63 ? starts >> (lane + 1) : 0ull;
                 const int end = later ? lane + 1 + __builtin_ctzll(later) : hi;
                 double a = my[dst], c = mi[dst];
-                for (int k = lane; k < end; ++k) {
-                    a += st_y[k];
-                    c += st_i[k];
+                for (int k = lane; k < end; k += 8) {
+                    double vy[8], vi[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int j = min(k + u, 63);
+                        vy[u] = st_y[j];
+                        vi[u] = st_i[j];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        if (k + u < end) {
+                            a += vy[u];
+                            c += vi[u];
+                        }
+                    }
                 }
                 my[dst] = a;
                 mi[dst] = c;
@@ -2921,11 +2942,56 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
                         ncyc <= (1 << 20) && lds_bins - T - mmax - 1 >= 64;
     if (!fits && !ranged && !(nb - os > os && lds_bins - mmax - 1 >= 1))
         continue;
-    const int S = fits ? nb + 1
-                       : (ranged ? lds_bins - T - mmax - 1
-                                 : lds_bins - mmax - 1);
-    int *lo_t = (int *)(my + (lds_bins - T));  // [2*ncyc] range starts
-    int *pre_t = lo_t + 2 * ncyc;               // [2*ncyc+1] item prefix
+    int S = fits ? nb + 1
+                 : (ranged ? lds_bins - T - mmax - 1 : lds_bins - mmax - 1);
+    // The range searches of all segments in one round, before the first
+    // (one chain of dependent loads per candidate instead of one per
+    // segment): a table of 2 * ncyc * nseg range bounds, plus the 2 * ncyc
+    // of bins 1..os, behind the per-segment ranges at the tail of my[],
+    // when it takes at most a quarter of it (the segments shrink to make
+    // room).
+    int Tt = T, nseg = 0;
+    bool tab_ok = false;
+    if (ranged) {
+        int ns = (nb + S - 1) / S;
+        for (int it = 0; it < 4; ++it) {
+            const long long ti = 6LL * ncyc + 1 + 2LL * ncyc * ns;
+            const int Tc = (int)((ti + 1) / 2) + 1;
+            const int Sc = lds_bins - Tc - mmax - 1;
+            if (Tc > lds_bins / 4 || Sc < 64) break;
+            const int nsc = (nb + Sc - 1) / Sc;
+            if (nsc <= ns) {
+                tab_ok = true;
+                Tt = Tc;
+                S = Sc;
+                nseg = ns;
+                break;
+            }
+            ns = nsc;
+        }
+    }
+    int *lo_t = (int *)(my + (lds_bins - Tt));  // [2*ncyc] range starts
+    int *pre_t = lo_t + 2 * ncyc;                // [2*ncyc+1] item prefix
+    int *tab = pre_t + 2 * ncyc + 1;             // [2*ncyc*(nseg+1)]
+    if (tab_ok) {
+        const int main_e = 2 * ncyc * nseg;
+        for (int e = lane; e < main_e + 2 * ncyc; e += 64) {
+            int v = 0;
+            if (e < main_e) {
+                const int j = e / (2 * ncyc), r = e - j * (2 * ncyc);
+                const int s0j = j * S, hij = min(s0j + S + mmax, nb);
+                if (s0j < nb)
+                    v = lower_key(tn, n, period, invP, bd, inv_bd, nb,
+                                  (double)(r >> 1), (r & 1) ? hij + 1 : s0j);
+            } else {
+                const int r = e - main_e;
+                v = lower_key(tn, n, period, invP, bd, inv_bd, nb,
+                              (double)(r >> 1), (r & 1) ? os + 1 : 1);
+            }
+            tab[e] = v;
+        }
+        __syncthreads();
+    }
     const bool like = b.objective == ATG_OBJ_LIKELIHOOD;
     const int nval = nb + 1;
     double cprev_y = 0.0, cprev_i = 0.0;
@@ -2946,15 +3012,18 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
         int cur = -1;
         if (ranged) {
             const bool extra = hi >= nb - os;
+            const int seg = s0 / S;
             for (int e = lane; e < 4 * ncyc; e += 64) {
                 const int k = e >> 1, c = k % ncyc;
                 int v = 0;
                 if (k < ncyc)
-                    v = lower_key(tn, n, period, invP, bd, inv_bd, nb,
-                                  (double)c, (e & 1) ? hi + 1 : s0);
+                    v = tab_ok ? tab[2 * ncyc * seg + 2 * c + (e & 1)]
+                               : lower_key(tn, n, period, invP, bd, inv_bd, nb,
+                                           (double)c, (e & 1) ? hi + 1 : s0);
                 else if (extra)
-                    v = lower_key(tn, n, period, invP, bd, inv_bd, nb,
-                                  (double)c, (e & 1) ? os + 1 : 1);
+                    v = tab_ok ? tab[2 * ncyc * nseg + 2 * c + (e & 1)]
+                               : lower_key(tn, n, period, invP, bd, inv_bd, nb,
+                                           (double)c, (e & 1) ? os + 1 : 1);
                 if (e & 1) pre_t[k] = v;  // range end, until the prefix
                 else lo_t[k] = v;
             }
@@ -3121,33 +3190,45 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
         __syncthreads();
         const double ty = s_tot[0], ti = s_tot[1];
         const int nend = min(s0 + S, nb);
-        for (int st = s0 + lane; st < nend; st += 64) {
-            const int l = st - s0;
-            for (int k = 0; k < nd; ++k) {
-                const int m = s_m[k];
-                if (st + m > nb) continue;
-                const double ys = my[l + m] - my[l];
-                const double is = mi[l + m] - mi[l];
-                double y_in = ys, iv_in = is;
-                double y_out = ty - y_in;
-                double iv_out = ti - iv_in;
-                if (iv_in < DBL_EPSILON || iv_out < DBL_EPSILON) continue;
-                y_in /= iv_in;
-                y_out /= iv_out;
-                const double depth = y_out - y_in;
-                const double derr = sqrt(1.0 / iv_in + 1.0 / iv_out);
-                const double snr = depth / derr;
-                const double ll = 0.5 * iv_in * depth * depth;
-                const double obj = like ? ll : snr;
-                const int idx = k * nval + st;
-                if (depth >= 0.0 && better(obj, idx, bo, bi)) {
-                    bo = obj;
-                    bi = idx;
-                    byin = ys;
-                    bivin = is;
+        // the oracle's formulas; a window of negative depth never wins, so
+        // its objective is not formed, and each objective forms only its
+        // own terms (the depth error and snr of the best window follow)
+        auto scan = [&](auto lk) {
+            constexpr bool LK = decltype(lk)::value;
+            for (int st = s0 + lane; st < nend; st += 64) {
+                const int l = st - s0;
+                for (int k = 0; k < nd; ++k) {
+                    const int m = s_m[k];
+                    if (st + m > nb) continue;
+                    const double ys = my[l + m] - my[l];
+                    const double is = mi[l + m] - mi[l];
+                    double y_in = ys, iv_in = is;
+                    double y_out = ty - y_in;
+                    double iv_out = ti - iv_in;
+                    if (iv_in < DBL_EPSILON || iv_out < DBL_EPSILON) continue;
+                    y_in /= iv_in;
+                    y_out /= iv_out;
+                    const double depth = y_out - y_in;
+                    if (!(depth >= 0.0)) continue;
+                    double obj;
+                    if (LK) {
+                        obj = 0.5 * iv_in * depth * depth;
+                    } else {
+                        const double derr = sqrt(1.0 / iv_in + 1.0 / iv_out);
+                        obj = depth / derr;
+                    }
+                    const int idx = k * nval + st;
+                    if (better(obj, idx, bo, bi)) {
+                        bo = obj;
+                        bi = idx;
+                        byin = ys;
+                        bivin = is;
+                    }
                 }
             }
-        }
+        };
+        if (like) scan(BoolC<true>());
+        else scan(BoolC<false>());
         __syncthreads();
     }
     for (int ofs = 32; ofs > 0; ofs >>= 1) {
