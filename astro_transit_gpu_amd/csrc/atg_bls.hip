@@ -3159,22 +3159,68 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
             double ci = s0 == 0 ? mi[0] : cprev_i + mi[0];
             my[0] = cy;
             mi[0] = ci;
-            // eight bins read ahead of their adds (the adds stay in order)
+            // bins read eight at a time, one group ahead of the adds of the
+            // other (the adds stay in order; reads past the end are clamped
+            // and unused; the scheduling barriers keep the groups apart)
             int l = 1;
-            for (; l + 8 <= len; l += 8) {
-                double vy[8], vi[8];
+            double ay8[8], ai8[8], by8[8], bi8[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                ay8[u] = my[min(l + u, len - 1)];
+                ai8[u] = mi[min(l + u, len - 1)];
+            }
+            for (; l + 16 <= len; l += 16) {
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    vy[u] = my[l + u];
-                    vi[u] = mi[l + u];
+                    by8[u] = my[l + 8 + u];
+                    bi8[u] = mi[l + 8 + u];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                // the adds of a group first, then its stores (a store waits
+                // for its add, and the wave issues in order)
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    cy = cy + ay8[u];
+                    ci = ci + ai8[u];
+                    ay8[u] = cy;
+                    ai8[u] = ci;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    my[l + u] = ay8[u];
+                    mi[l + u] = ai8[u];
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    cy = cy + vy[u];
-                    ci = ci + vi[u];
+                    ay8[u] = my[min(l + 16 + u, len - 1)];
+                    ai8[u] = mi[min(l + 16 + u, len - 1)];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    cy = cy + by8[u];
+                    ci = ci + bi8[u];
+                    by8[u] = cy;
+                    bi8[u] = ci;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    my[l + 8 + u] = by8[u];
+                    mi[l + 8 + u] = bi8[u];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (l + 8 <= len) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    cy = cy + ay8[u];
+                    ci = ci + ai8[u];
                     my[l + u] = cy;
                     mi[l + u] = ci;
                 }
+                l += 8;
             }
             for (; l < len; ++l) {
                 cy = cy + my[l];
