@@ -729,6 +729,10 @@ __device__ __forceinline__ void quantize_raw(double y, double d, double med,
 template <class R>
 __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     __shared__ unsigned int hist[512];
+    // the sequential totals' chunks: two buffers of SEQ_CH (term, weight)
+    // pairs, each padded by 8 for the summing wave's read-ahead
+    constexpr int SEQ_CH = 3 * 64 * 4;
+    __shared__ double2 seq_v[2 * (SEQ_CH + 8)];
     __shared__ int sel[4];
     __shared__ unsigned int uscr[NW];
     __shared__ double dscr[NW];
@@ -838,6 +842,84 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     select2<R>(b, off, n, (n - 1) / 2, n / 2, &v0, &v1, hist, sel, uscr);
     const double med = (n & 1) ? v0 : (v0 + v1) * 0.5;
     P.y_med = med;
+
+    // parity mode: the oracle's FP64 totals in its order (the refinement
+    // kernel uses them).  Each total is one chain of dependent FP64 adds
+    // (~20 cycles each), so one wave adds while the others stage: waves 1-3
+    // put the next chunk of SEQ_CH (term, weight) pairs in LDS while wave 0
+    // adds up the current one in point order from broadcast reads, eight
+    // pairs read ahead of their adds.
+    if (b.mode == ATG_MODE_PARITY) {
+        constexpr int SPAN = 3 * 64;  // staging threads
+        const int nch = (n + SEQ_CH - 1) / SEQ_CH;
+        auto stage = [&](int c) {
+            double2 *q = seq_v + (c & 1) * (SEQ_CH + 8);
+            const int j = tid - 64, c0 = c * SEQ_CH;
+            double yr[4], dr[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                load_point<R>(b, off + min(c0 + u * SPAN + j, n - 1), yr[u],
+                              dr[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double iv = R::HAS_DY ? 1.0 / (dr[u] * dr[u]) : 1.0;
+                q[u * SPAN + j] = make_double2((yr[u] - med) * iv, iv);
+            }
+        };
+        double sy = 0.0, si = 0.0;
+        if (tid >= 64) stage(0);
+        __syncthreads();
+        for (int c = 0; c < nch; ++c) {
+            if (tid >= 64) {
+                if (c + 1 < nch) stage(c + 1);
+            } else {
+                const double2 *q = seq_v + (c & 1) * (SEQ_CH + 8);
+                const int cnt = min(SEQ_CH, n - c * SEQ_CH);
+                const int full = cnt & ~7;
+                // reads reach index full + 7 at most: within the padding;
+                // the scheduling barriers keep each group's reads ahead of
+                // the other group's adds
+                double2 A[8], B[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) A[u] = q[u];
+                int k = 0;
+                for (; k + 16 <= full; k += 16) {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) B[u] = q[k + 8 + u];
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        sy = sy + A[u].x;
+                        si = si + A[u].y;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) A[u] = q[k + 16 + u];
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        sy = sy + B[u].x;
+                        si = si + B[u].y;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                if (k + 8 <= full) {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        sy = sy + A[u].x;
+                        si = si + A[u].y;
+                    }
+                    k += 8;
+                }
+                for (; k < cnt; ++k) {
+                    sy = sy + q[k].x;
+                    si = si + q[k].y;
+                }
+            }
+            __syncthreads();
+        }
+        if (tid == 0) w.seq[lc] = make_double2(sy, si);
+    }
 
     // pass B: magnitudes for the fixed-point scales, weight uniformity, and
     // the exponent histogram of the flux magnitudes
@@ -1106,59 +1188,6 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     P.W_tot = (double)wq_sum * P.w_unit;
     P.Y_tot = (double)yq_sum * P.y_unit;
     if (tid == 0) w.prm[lc] = P;
-}
-
-// ------------------------------------------------------------------- k_seq
-
-// Parity mode: the oracle's FP64 totals of the curve in its order (sum of
-// (y - median) * ivar and sum of ivar, point after point; the refinement
-// kernel needs them).  Each total is one chain of dependent FP64 adds (~20
-// cycles each), so one wave per curve walks them, on a stream of its own
-// beside the periodogram kernels.  The wave holds SEQ_U chunks of 64 terms
-// in registers and adds them lane by lane (a lane's value read into scalar
-// registers), while the next group's raw points load; it uses no LDS, which
-// the periodogram kernels keep busy.
-constexpr int SEQ_U = 4, SEQ_G = 64 * SEQ_U;
-
-template <class R>
-__global__ __launch_bounds__(64) void k_seq(Batch b, Work w) {
-    const int lc = blockIdx.x;
-    if (b.mode != ATG_MODE_PARITY || w.prm[lc].status != ATG_LC_OK) return;
-    const long long off = b.lc_off[lc];
-    const int n = w.prm[lc].n;
-    const double med = w.prm[lc].y_med;
-    const int lane = threadIdx.x;
-    double yr[SEQ_U], dr[SEQ_U];
-    auto load = [&](int g0) {
-#pragma unroll
-        for (int u = 0; u < SEQ_U; ++u) {
-            const int i = min(g0 + u * 64 + lane, n - 1);
-            yr[u] = R::y(b, off + i);
-            dr[u] = R::dy(b, off + i);
-        }
-    };
-    load(0);
-    double sy = 0.0, si = 0.0;
-    for (int g0 = 0; g0 < n; g0 += SEQ_G) {
-        double ty[SEQ_U], tw[SEQ_U];
-#pragma unroll
-        for (int u = 0; u < SEQ_U; ++u) {
-            const double iv = R::HAS_DY ? 1.0 / (dr[u] * dr[u]) : 1.0;
-            ty[u] = (yr[u] - med) * iv;
-            tw[u] = iv;
-        }
-        if (g0 + SEQ_G < n) load(g0 + SEQ_G);
-#pragma unroll
-        for (int u = 0; u < SEQ_U; ++u) {
-            const int cnt = min(64, n - (g0 + u * 64));
-            if (cnt <= 0) break;
-            for (int j = 0; j < cnt; ++j) {
-                sy = sy + readlane_d(ty[u], j);
-                si = si + readlane_d(tw[u], j);
-            }
-        }
-    }
-    if (lane == 0) w.seq[lc] = make_double2(sy, si);
 }
 
 // ----------------------------------------------------------------- k_items
@@ -3940,10 +3969,6 @@ void *scratch_on(int device, hipError_t *err) {
 struct atg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // parity mode's sequential totals (k_seq) run on a stream of their own,
-    // forked after k_prep and joined before k_exact
-    hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     uint64_t budget = 0;
     std::string err;
     void *ws = nullptr;
@@ -4201,18 +4226,6 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
         hipLaunchKernelGGL(k_prep<decltype(r)>, dim3(nlc), dim3(NT), 0,
                            c->stream, kb, w, c->tune.lds_bins);
     });
-    const bool par = b->mode == ATG_MODE_PARITY;
-    if (par) {
-        e = hipEventRecord(c->ev_fork, c->stream);
-        if (e == hipSuccess) e = hipStreamWaitEvent(c->side, c->ev_fork, 0);
-        if (e != hipSuccess) return hip_fail(c, e, "fork of the totals' stream");
-        with_raw_types(kb, [&](auto r) {
-            hipLaunchKernelGGL(k_seq<decltype(r)>, dim3(nlc), dim3(64), 0,
-                               c->side, kb, w);
-        });
-        e = hipEventRecord(c->ev_join, c->side);
-        if (e != hipSuccess) return hip_fail(c, e, "join of the totals' stream");
-    }
     hipLaunchKernelGGL(k_items, dim3(1), dim3(NT), 0, c->stream,
                        (long long)b->n_lc, kb.per_off, per_item, w);
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -4254,10 +4267,6 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
     }
     hipLaunchKernelGGL(k_cands, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w);
     const int ex_bins = std::min(c->tune.lds_bins, EXACT_BINS);
-    if (par) {
-        e = hipStreamWaitEvent(c->stream, c->ev_join, 0);
-        if (e != hipSuccess) return hip_fail(c, e, "join of the totals' stream");
-    }
     with_raw_types(kb, [&](auto r) {
         hipLaunchKernelGGL(k_exact<decltype(r)>,
                            dim3(c->num_cu * EXACT_WG_PER_CU), dim3(64),
@@ -4297,25 +4306,6 @@ int atg_ctx_create(int device, void *stream, uint64_t hbm_budget_bytes,
     atg_ctx *c = new atg_ctx();
     c->device = device;
     c->stream = (hipStream_t)stream;
-    {
-        int prev = 0;
-        hipGetDevice(&prev);
-        hipError_t e = hipSetDevice(device);
-        if (e == hipSuccess)
-            e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
-        if (e == hipSuccess)
-            e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
-        if (e == hipSuccess)
-            e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
-        hipSetDevice(prev);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            if (c->ev_fork) hipEventDestroy(c->ev_fork);
-            if (c->side) hipStreamDestroy(c->side);
-            delete c;
-            return ATG_ERR_HIP;
-        }
-    }
     c->budget = hbm_budget_bytes;
     c->num_cu = prop.multiProcessorCount;
     c->lds_max = prop.sharedMemPerBlock;
@@ -4333,10 +4323,6 @@ void atg_ctx_destroy(atg_ctx *c) {
         hipEventDestroy(pr.second);
     }
     for (auto e : c->spare) hipEventDestroy(e);
-    if (c->side) hipStreamSynchronize(c->side);
-    if (c->ev_fork) hipEventDestroy(c->ev_fork);
-    if (c->ev_join) hipEventDestroy(c->ev_join);
-    if (c->side) hipStreamDestroy(c->side);
     if (c->ws) hipFree(c->ws);
     delete c;
 }
