@@ -20,6 +20,8 @@ cli.py:465-466, scripts/sector_marathon_v3.py:24-28) is the "plain" form
 this replaces.
 """
 import collections
+import concurrent.futures
+import os
 import time
 from dataclasses import dataclass, field
 from typing import List, Optional
@@ -32,6 +34,13 @@ from .engine import FIELDS, Context, DeviceBatch, DeviceResult
 def _al(n):
     # requests sit back to back: a batch's CSR ranges must be contiguous
     return n
+
+
+# Host copies into the pinned staging buffer are split into pieces of about
+# this many bytes and run on worker threads (numpy releases the GIL while it
+# copies plain arrays); a round of admitted requests is tens of MB.
+_STAGE_PIECE = 4 << 20
+_STAGE_THREADS = max(1, min(8, (os.cpu_count() or 1) // 2))
 
 
 @dataclass
@@ -55,6 +64,14 @@ class Request:
     @property
     def latency(self):
         return self.done - self.arrival
+
+
+def _fill(piece):
+    out, arrs = piece
+    if len(arrs) == 1:
+        np.copyto(out, arrs[0], casting="no")
+    else:
+        np.concatenate(arrs, out=out, casting="no")
 
 
 class _Ring:
@@ -149,6 +166,7 @@ class BLSServer:
             self.slots.append(dict(stream=st, ctx=Context(dev, stream=st,
                                                           tuning=tuning),
                                    busy=None))
+        self._pool = None                     # host staging threads
         self.pending = collections.deque()    # submitted, not admitted
         self.admitted = collections.deque()   # admitted, not launched
         self.inflight = collections.deque()   # launched batches (FIFO)
@@ -196,37 +214,68 @@ class BLSServer:
         if not round_:
             return 0
         with torch.cuda.stream(self.copy_stream):
-            self._stage(round_, "pt", self.a_t, lambda r: r.t, np.float64)
-            self._stage(round_, "pt", self.a_y, lambda r: r.y, self.y_dtype)
-            self._stage([r for r in round_ if r.dy is not None], "pt",
-                        self.a_dy, lambda r: r.dy, self.y_dtype)
-            self._stage(round_, "pp", self.a_p, lambda r: r.periods,
-                        np.float64)
-            self._stage(round_, "pd", self.a_d, lambda r: r.durations,
-                        np.float64)
+            self._stage(round_)
         self.admitted.extend(round_)
         return len(round_)
 
-    def _stage(self, reqs, attr, dst, get, dtype):
-        i = 0
-        while i < len(reqs):
-            j = i + 1
-            end = getattr(reqs[i], attr) + get(reqs[i]).size
-            while j < len(reqs) and getattr(reqs[j], attr) == end:
-                end += get(reqs[j]).size
-                j += 1
-            start = getattr(reqs[i], attr)
-            n = end - start
-            if n:
-                pin = torch.empty(n, dtype=dst.dtype, pin_memory=True)
-                host = pin.numpy()
-                o = 0
-                for r in reqs[i:j]:
-                    a = get(r)
-                    host[o:o + a.size] = a
-                    o += a.size
-                dst[start:end].copy_(pin, non_blocking=True)
-            i = j
+    def _stage(self, round_):
+        """Copy a round's arrays to their rings: every contiguous run of a
+        ring (a wrap splits a run) is packed into one pinned host buffer for
+        the whole round, by worker threads, then copied with one H2D copy."""
+        runs = []  # (ring tensor, start, arrays, numpy dtype)
+        specs = [(self.a_t, "pt", lambda r: r.t, np.float64, round_),
+                 (self.a_y, "pt", lambda r: r.y, self.y_dtype, round_),
+                 (self.a_dy, "pt", lambda r: r.dy, self.y_dtype,
+                  [r for r in round_ if r.dy is not None]),
+                 (self.a_p, "pp", lambda r: r.periods, np.float64, round_),
+                 (self.a_d, "pd", lambda r: r.durations, np.float64, round_)]
+        for dst, attr, get, dtype, reqs in specs:
+            i = 0
+            while i < len(reqs):
+                j = i + 1
+                end = getattr(reqs[i], attr) + get(reqs[i]).size
+                while j < len(reqs) and getattr(reqs[j], attr) == end:
+                    end += get(reqs[j]).size
+                    j += 1
+                arrs = [get(r) for r in reqs[i:j]]
+                if end > getattr(reqs[i], attr):
+                    runs.append((dst, getattr(reqs[i], attr), arrs,
+                                 np.dtype(dtype)))
+                i = j
+        if not runs:
+            return
+        offs, total = [], 0
+        for _, start, arrs, dt in runs:
+            offs.append(total)
+            total += (sum(a.size for a in arrs) * dt.itemsize + 63) & ~63
+        pin = torch.empty(total, dtype=torch.uint8, pin_memory=True)
+        host = pin.numpy()
+        pieces = []  # (output view, arrays): contiguous sub-ranges of a run
+        for (_, _, arrs, dt), off in zip(runs, offs):
+            n = sum(a.size for a in arrs)
+            out = host[off:off + n * dt.itemsize].view(dt)
+            o, k = 0, 0
+            while k < len(arrs):
+                k0, o0, nb = k, o, 0
+                while k < len(arrs) and (nb == 0 or nb < _STAGE_PIECE):
+                    nb += arrs[k].size * dt.itemsize
+                    o += arrs[k].size
+                    k += 1
+                pieces.append((out[o0:o], arrs[k0:k]))
+        if total >= 2 * _STAGE_PIECE and _STAGE_THREADS > 1:
+            if self._pool is None:
+                self._pool = concurrent.futures.ThreadPoolExecutor(
+                    max_workers=_STAGE_THREADS)
+            list(self._pool.map(_fill, pieces))
+        else:
+            for pc in pieces:
+                _fill(pc)
+        tdt = {np.dtype(np.float64): torch.float64,
+               np.dtype(np.float32): torch.float32}
+        for (dst, start, arrs, dt), off in zip(runs, offs):
+            n = sum(a.size for a in arrs)
+            src = pin[off:off + n * dt.itemsize].view(tdt[dt])
+            dst[start:start + n].copy_(src, non_blocking=True)
 
     # ------------------------------------------------------------ batching
     def _take_batch(self):
@@ -284,8 +333,10 @@ class BLSServer:
                                period_offsets=po)
             slot["ctx"].run(batch, objective=self.objective, mode=self.mode,
                             oversample=self.oversample, out=out, sync=False)
-            host = {k_: torch.empty(np_, dtype=torch.float64,
-                                    pin_memory=True) for k_ in FIELDS}
+            # the seven result arrays in one pinned buffer, one row each
+            hbuf = torch.empty((len(FIELDS), np_), dtype=torch.float64,
+                               pin_memory=True)
+            host = {k_: hbuf[i_] for i_, k_ in enumerate(FIELDS)}
             for k_ in FIELDS:
                 host[k_].copy_(getattr(out, k_), non_blocking=True)
             bi = torch.empty(k, dtype=torch.int64, pin_memory=True)

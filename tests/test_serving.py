@@ -77,3 +77,34 @@ def test_served_in_mix_equals_alone(mode):
         if mode == "parity" and i < 6:
             ref = O.power(r[0], r[1], r[2], r[3], r[4], "likelihood")
             assert_parity(h.result, ref, "request %d" % i)
+
+
+@pytest.mark.gpu
+def test_served_large_rounds_equal_alone():
+    """Rounds of admitted requests large enough for the threaded staging
+    (tens of MB of points in one pinned buffer, copied in pieces by worker
+    threads): every request's result equals its alone run bit for bit."""
+    from astro_transit_gpu_amd import Context, DeviceBatch, synth
+    from astro_transit_gpu_amd.serving import BLSServer
+    rng = np.random.default_rng(17)
+    reqs = []
+    for _ in range(12):
+        t, y, dy = synth.ragged_curve(rng, 120000, 200000, 20.0, 60.0)
+        durs = np.array([0.05, 0.1, 0.2])
+        per = synth.capped_grid(t, durs, 40, 200)
+        reqs.append((t, y.astype(np.float32), dy.astype(np.float32), per,
+                     durs))
+    srv = BLSServer(objective="likelihood", max_batch=4,
+                    capacity_points=4000000, capacity_periods=10000,
+                    capacity_durations=200)
+    handles = [srv.submit(*r) for r in reqs]  # one round of ~40 MB
+    srv.drain()
+    ctx = Context(0)
+    for i, (h, r) in enumerate(zip(handles, reqs)):
+        assert h.status == 0
+        b = DeviceBatch.from_host([r[0]], [r[1]], [r[2]], r[3], r[4],
+                                  device=ctx.device)
+        alone = ctx.run(b, objective="likelihood").to_host()
+        for k in h.result:
+            assert np.array_equal(h.result[k], alone[k]), (i, k)
+        assert h.best_index == int(alone["best_index"][0])
