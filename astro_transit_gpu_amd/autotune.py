@@ -33,7 +33,7 @@ STATIC_LDS = 256           # the main kernel's static LDS
 MAX_WAVES_PER_SIMD = 4     # register-limited
 # bumped whenever the kernel's speed profile changes, so that cached
 # tilings of an older build are not reused
-KERNEL_GENERATION = 6
+KERNEL_GENERATION = 7
 
 
 def candidates(lds_options=(2048, 2432, 2540, 3072, 3392, 4096, 5120, 6144),
