@@ -60,7 +60,13 @@ constexpr int NT = 256;      // threads per workgroup = 4 wave64
 constexpr int NW = NT / 64;  // waves per workgroup
 constexpr int SUBMAX = 8;    // periods folded together in one pass
 constexpr int DMAX = 64;     // durations per light curve
-constexpr int EXACT_BINS = 1024;
+// k_exact: EXW waves per workgroup, whose LDS holds EXACT_BINS FP64 bins
+// (at most; the tuning's lds_bins caps it), EXW regions of equal size
+constexpr int EXW = 4;
+constexpr int EXACT_BINS = 1920;
+constexpr int exact_total(int lds_bins) {
+    return (lds_bins < EXACT_BINS ? lds_bins : EXACT_BINS) & ~(EXW - 1);
+}
 constexpr int NQ = 8;        // work queues = XCDs of an MI355X
 constexpr int EHIST = 1075 + 1024;  // k_prep's exponent histogram
 // entries past a period's prefix arrays that k_main's box scan reads (the
@@ -1139,7 +1145,7 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
     // curves only): of k_big, for periods whose bins exceed k_main's
     // histogram (packed bins take 12 bytes each in its scan, 16 otherwise,
     // out of lds_bins * 16), and of k_exact in parity mode, whose histogram
-    // holds at most EXACT_BINS bins
+    // holds at most exact_total(lds_bins) bins
     P.natural = 0;
     const double nbmax = ceil(pmax / P.bd) + b.oversample;
     // as k_main's capacity test for the longest period alone (with the
@@ -1152,7 +1158,7 @@ __global__ __launch_bounds__(NT) void k_prep(Batch b, Work w, int lds_bins) {
                  : nbmax + 1 + ov > (double)(P.uniform ? counted_cap(lds_bins)
                                                        : lds_bins);
     const bool big_exact = b.mode == ATG_MODE_PARITY &&
-                           nbmax + 1 > (double)min(lds_bins, EXACT_BINS);
+                           nbmax + 1 > (double)exact_total(lds_bins);
     if (P.sorted && (big || big_exact)) {
         P.natural = 1;
         const long long nbase = nat_base(b, lc);
@@ -2738,10 +2744,12 @@ namespace {
 // member (exact integer value) could outrank the refined ones.  The queue
 // holds up to every period of the batch, so nothing is dropped.
 constexpr double TIE_REL = 1e-9;   // >> relative error of the int64 path
-// k_exact's FP64 histogram: one wave per queued period walks its points
-// sequentially, so the kernel is latency-bound; a small LDS histogram (16 KB)
-// lets 8 such waves share a CU (longer periods take the ranged segments).
-constexpr int EXACT_WG_PER_CU = 8;
+// k_exact's workgroups: the sums are chains in point order, so the kernel is
+// latency-bound; each queued period gets EXW waves (a quarter of its bins,
+// or a segment, each), three workgroups to a CU (its ~170 VGPRs allow three
+// waves per SIMD; capped at 128 it spills and runs slower).
+constexpr int EXACT_WG_PER_CU = 3;
+constexpr int EXACT_TAB = 1024;  // the ranged form's table of range bounds
 
 __global__ __launch_bounds__(NT) void k_cands(Batch b, Out o, Work w) {
     __shared__ double dscr[NW];
@@ -2917,20 +2925,123 @@ __device__ __forceinline__ void exact_add_chunk(int dst, double yv, double iv,
     }
 }
 
+// Sequential inclusive prefix sums of y[0, len) and v[0, len) in place (one
+// thread), continuing from the previous segment's totals (cy, cv) unless
+// first.  Bins are read eight at a time, one group ahead of the other
+// group's adds, and a group is stored after its adds (a store waits for its
+// add, and the wave issues in order); the reads run up to 7 past len, into
+// the next region or the arrays' padding, and are unused there.
+__device__ __forceinline__ void exact_prefix(double *y, double *v, int len,
+                                             bool first, double cy,
+                                             double cv) {
+    cy = first ? y[0] : cy + y[0];
+    cv = first ? v[0] : cv + v[0];
+    y[0] = cy;
+    v[0] = cv;
+    int l = 1;
+    double ay[8], av[8], by[8], bv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        ay[u] = y[l + u];
+        av[u] = v[l + u];
+    }
+    for (; l + 16 <= len; l += 16) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            by[u] = y[l + 8 + u];
+            bv[u] = v[l + 8 + u];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            cy = cy + ay[u];
+            cv = cv + av[u];
+            ay[u] = cy;
+            av[u] = cv;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            y[l + u] = ay[u];
+            v[l + u] = av[u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            ay[u] = y[l + 16 + u];
+            av[u] = v[l + 16 + u];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            cy = cy + by[u];
+            cv = cv + bv[u];
+            by[u] = cy;
+            bv[u] = cv;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            y[l + 8 + u] = by[u];
+            v[l + 8 + u] = bv[u];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (l + 8 <= len) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            cy = cy + ay[u];
+            cv = cv + av[u];
+            y[l + u] = cy;
+            v[l + u] = cv;
+        }
+        l += 8;
+    }
+    for (; l < len; ++l) {
+        cy = cy + y[l];
+        cv = cv + v[l];
+        y[l] = cy;
+        v[l] = cv;
+    }
+}
+
+// One queued period recomputed by a workgroup of EXW waves.  In LDS: my and
+// mi (sum of y*ivar and of ivar per bin), EXW regions of rb bins each, plus 8
+// bins of padding for the prefix's read-ahead.  Three forms:
+// - in LDS (nb + 1 <= EXW * rb): every wave walks every point in point order
+//   and adds those whose bin lies in its quarter of the bins, so each bin
+//   still receives its points in point order; then the padding assignment,
+//   the sequential prefix (one thread) and the box scan over all threads;
+// - ranged (time-sorted curves with point-order copies, longer periods):
+//   segments of S start positions (plus the longest duration's halo), one
+//   per wave and region at a time; a wave walks only its segment's points
+//   (the ranges of every cycle, then those of bins 1..os whose padding copies
+//   land in the segment); after each round of EXW segments one thread runs
+//   the prefix through them in order, carrying the total from segment to
+//   segment, and each wave scans its segment.  The ranges of all segments
+//   are searched in one round before the first when their table fits;
+// - otherwise (unsorted curves, longer periods): segments over the whole
+//   LDS, the first wave walking every point with its padded destinations.
+// Outputs are the oracle's, as before: each bin's FP64 sum in point order,
+// the oracle's padding, prefix order and formulas; the best window is the
+// maximum over all of them by (objective, then lowest index).
 template <class R>
-__global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
-                                              int lds_bins) {
+__global__ __launch_bounds__(64 * EXW) void k_exact(Batch b, Out o, Work w,
+                                                    int rb) {
     extern __shared__ __align__(16) unsigned char smem[];
-    double *my = (double *)smem;        // sum of y*ivar per bin
-    double *mi = my + lds_bins;         // sum of ivar per bin
+    const int lds_bins = EXW * rb;
+    double *my = (double *)smem;         // sum of y*ivar per bin
+    double *mi = my + lds_bins + 8;      // sum of ivar per bin
     __shared__ int s_m[DMAX];
     __shared__ double s_tot[2];
-    __shared__ double st_y[64], st_i[64];  // a chunk's values, staged
+    __shared__ double st_y[EXW][64], st_i[EXW][64];  // chunk values, staged
+    __shared__ int s_tab[EXACT_TAB];
+    __shared__ double r_bo[EXW], r_y[EXW], r_i[EXW];
+    __shared__ int r_bi[EXW];
     __shared__ unsigned int s_q;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     for (;;) {
     __syncthreads();
-    if (lane == 0) {
+    if (tid == 0) {
         // the drained queue costs a load, not an atomic on its one counter
         const unsigned int cnt = *w.cand_count;
         s_q = __hip_atomic_load(w.cand_next, __ATOMIC_RELAXED,
@@ -2950,173 +3061,85 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
     const int nb = (int)ceil(period / bd) + os;
     const long long off = b.lc_off[lc];
     const int n = P.n, nd = P.n_dur;
-    if (lane < nd) s_m[lane] = (int)round(b.durs[b.dur_off[lc] + lane] / bd);
+    if (tid < nd) s_m[tid] = (int)round(b.durs[b.dur_off[lc] + tid] / bd);
+    if (tid == 0) {
+        s_tot[0] = w.seq[lc].x;
+        s_tot[1] = w.seq[lc].y;
+    }
     __syncthreads();
     int mmax = 0;
     for (int k = 0; k < nd; ++k) mmax = max(mmax, s_m[k]);
-    // in LDS: raw bins, then the oracle's sequential padding; otherwise
-    // segments of start positions with the padded destinations of each
-    // point and the exact FP64 prefix carried from segment to segment.
-    // Segments of time-sorted curves walk only their own points: the
-    // ranges of every cycle (as in k_main's phase tiles), then the ranges
-    // of bins 1..os whose copies land in the segment.  Each bin still
-    // receives its points in point order.
     const bool fits = nb + 1 <= lds_bins;
     const double *tn = w.nat_ty + 2 * nat_base(b, lc);
     int ncyc = 0;
     if (!fits && P.natural)
         ncyc = (int)fold_cycle(tn[2 * nat_slot(n - 1)], period, invP) + 1;
-    const int T = 2 * ncyc + 2;  // tail slots of my[] for the range table
+    const int T = 2 * ncyc + 2;  // a region's tail (doubles): its ranges
     const bool ranged = !fits && P.natural && nb - os > os &&
-                        ncyc <= (1 << 20) && lds_bins - T - mmax - 1 >= 64;
+                        ncyc <= (1 << 20) && rb - T - mmax - 1 >= 64;
     if (!fits && !ranged && !(nb - os > os && lds_bins - mmax - 1 >= 1))
         continue;
-    int S = fits ? nb + 1
-                 : (ranged ? lds_bins - T - mmax - 1 : lds_bins - mmax - 1);
-    // The range searches of all segments in one round, before the first
-    // (one chain of dependent loads per candidate instead of one per
-    // segment): a table of 2 * ncyc * nseg range bounds, plus the 2 * ncyc
-    // of bins 1..os, behind the per-segment ranges at the tail of my[],
-    // when it takes at most a quarter of it (the segments shrink to make
-    // room).
-    int Tt = T, nseg = 0;
-    bool tab_ok = false;
-    if (ranged) {
-        int ns = (nb + S - 1) / S;
-        for (int it = 0; it < 4; ++it) {
-            const long long ti = 6LL * ncyc + 1 + 2LL * ncyc * ns;
-            const int Tc = (int)((ti + 1) / 2) + 1;
-            const int Sc = lds_bins - Tc - mmax - 1;
-            if (Tc > lds_bins / 4 || Sc < 64) break;
-            const int nsc = (nb + Sc - 1) / Sc;
-            if (nsc <= ns) {
-                tab_ok = true;
-                Tt = Tc;
-                S = Sc;
-                nseg = ns;
-                break;
-            }
-            ns = nsc;
-        }
-    }
-    int *lo_t = (int *)(my + (lds_bins - Tt));  // [2*ncyc] range starts
-    int *pre_t = lo_t + 2 * ncyc;                // [2*ncyc+1] item prefix
-    int *tab = pre_t + 2 * ncyc + 1;             // [2*ncyc*(nseg+1)]
-    if (tab_ok) {
-        const int main_e = 2 * ncyc * nseg;
-        for (int e = lane; e < main_e + 2 * ncyc; e += 64) {
-            int v = 0;
-            if (e < main_e) {
-                const int j = e / (2 * ncyc), r = e - j * (2 * ncyc);
-                const int s0j = j * S, hij = min(s0j + S + mmax, nb);
-                if (s0j < nb)
-                    v = lower_key(tn, n, period, invP, bd, inv_bd, nb,
-                                  (double)(r >> 1), (r & 1) ? hij + 1 : s0j);
-            } else {
-                const int r = e - main_e;
-                v = lower_key(tn, n, period, invP, bd, inv_bd, nb,
-                              (double)(r >> 1), (r & 1) ? os + 1 : 1);
-            }
-            tab[e] = v;
-        }
-        __syncthreads();
-    }
+    const int S = fits ? nb + 1
+                       : (ranged ? rb - T - mmax - 1 : lds_bins - mmax - 1);
     const bool like = b.objective == ATG_OBJ_LIKELIHOOD;
     const int nval = nb + 1;
-    double cprev_y = 0.0, cprev_i = 0.0;
+    const double ty = s_tot[0], ti = s_tot[1];
     double bo = -INFINITY, byin = 0.0, bivin = 0.0;
     int bi = INT_MAX;
-    if (lane == 0) {
-        s_tot[0] = w.seq[lc].x;
-        s_tot[1] = w.seq[lc].y;
-    }
-    for (int s0 = 0; s0 < nb; s0 += S) {
-        const int hi = fits ? nb : min(s0 + S + mmax, nb);
-        const int len = hi - s0 + 1;
-        for (int i = lane; i < len; i += 64) {
+    // box scan of starts st0, st0 + stride, ... below nend over the prefix
+    // values of the segment at s0 held in (y, v): the oracle's formulas; a
+    // window of negative depth never wins, so its objective is not formed,
+    // and each objective forms only its own terms (the depth error and snr
+    // of the best window follow)
+    auto scan = [&](auto lk, const double *y, const double *v, int s0,
+                    int st0, int nend, int stride) {
+        constexpr bool LK = decltype(lk)::value;
+        for (int st = st0; st < nend; st += stride) {
+            const int l = st - s0;
+            for (int k = 0; k < nd; ++k) {
+                const int m = s_m[k];
+                if (st + m > nb) continue;
+                const double ys = y[l + m] - y[l];
+                const double is = v[l + m] - v[l];
+                double y_in = ys, iv_in = is;
+                double y_out = ty - y_in;
+                double iv_out = ti - iv_in;
+                if (iv_in < DBL_EPSILON || iv_out < DBL_EPSILON) continue;
+                y_in /= iv_in;
+                y_out /= iv_out;
+                const double depth = y_out - y_in;
+                if (!(depth >= 0.0)) continue;
+                double obj;
+                if (LK) {
+                    obj = 0.5 * iv_in * depth * depth;
+                } else {
+                    const double derr = sqrt(1.0 / iv_in + 1.0 / iv_out);
+                    obj = depth / derr;
+                }
+                const int idx = k * nval + st;
+                if (better(obj, idx, bo, bi)) {
+                    bo = obj;
+                    bi = idx;
+                    byin = ys;
+                    bivin = is;
+                }
+            }
+        }
+    };
+    auto scan_seg = [&](const double *y, const double *v, int s0, int st0,
+                        int nend, int stride) {
+        if (like) scan(BoolC<true>(), y, v, s0, st0, nend, stride);
+        else scan(BoolC<false>(), y, v, s0, st0, nend, stride);
+    };
+    if (fits) {
+        for (int i = tid; i <= nb; i += 64 * EXW) {
             my[i] = 0.0;
             mi[i] = 0.0;
         }
-        double ay = 0.0, ai = 0.0;
-        int cur = -1;
-        if (ranged) {
-            const bool extra = hi >= nb - os;
-            const int seg = s0 / S;
-            for (int e = lane; e < 4 * ncyc; e += 64) {
-                const int k = e >> 1, c = k % ncyc;
-                int v = 0;
-                if (k < ncyc)
-                    v = tab_ok ? tab[2 * ncyc * seg + 2 * c + (e & 1)]
-                               : lower_key(tn, n, period, invP, bd, inv_bd, nb,
-                                           (double)c, (e & 1) ? hi + 1 : s0);
-                else if (extra)
-                    v = tab_ok ? tab[2 * ncyc * nseg + 2 * c + (e & 1)]
-                               : lower_key(tn, n, period, invP, bd, inv_bd, nb,
-                                           (double)c, (e & 1) ? os + 1 : 1);
-                if (e & 1) pre_t[k] = v;  // range end, until the prefix
-                else lo_t[k] = v;
-            }
-            __syncthreads();
-            if (lane == 0) {
-                int acc = 0;
-                for (int k = 0; k < 2 * ncyc; ++k) {
-                    const int l = pre_t[k] - lo_t[k];
-                    pre_t[k] = acc;
-                    acc += l;
-                }
-                pre_t[2 * ncyc] = acc;
-            }
-            __syncthreads();
-            const int total = pre_t[2 * ncyc];
-            // item -> (range, point): a lane's items rise by 64 from one
-            // chunk to the next, so its range cursor only moves forward (one
-            // LDS read a chunk, more where ranges end).  The raw points of
-            // the next EXACT_AHEAD chunks are in flight while the wave walks
-            // the current one.
-            int kc = 0;
-            auto fetch = [&](int item, ExactPt &r) {
-                if (item < total) {
-                    while (pre_t[kc + 1] <= item) ++kc;
-                    const int j = lo_t[kc] + (item - pre_t[kc]);
-                    r.k = kc;
-                    r.t = b.t[off + j];
-                    r.y = R::y(b, off + j);
-                    r.d = R::dy(b, off + j);
-                }
-            };
-            ExactPt q[EXACT_AHEAD];
-#pragma unroll
-            for (int u = 0; u < EXACT_AHEAD; ++u) fetch(64 * u + lane, q[u]);
-            for (int cb0 = 0; cb0 < total; cb0 += 64 * EXACT_AHEAD) {
-#pragma unroll
-                for (int u = 0; u < EXACT_AHEAD; ++u) {
-                    const int cb = cb0 + 64 * u;
-                    if (cb >= total) break;
-                    const ExactPt r = q[u];
-                    fetch(cb + 64 * EXACT_AHEAD + lane, q[u]);
-                    const int k = r.k;
-                    const bool valid = cb + lane < total;
-                    const double tp = r.t - P.t_ref;
-                    const double iv = R::HAS_DY ? 1.0 / (r.d * r.d) : 1.0;
-                    const double yv = (r.y - P.y_med) * iv;
-                    int loc = -1;
-                    if (valid) {
-                        const int c = k < ncyc ? k : k - ncyc;
-                        const int rb = fold_bin_in(tp, period, (double)c, bd,
-                                                   inv_bd, nb);
-                        int dst;
-                        if (k < ncyc) dst = (rb < nb - os || rb == nb) ? rb : -1;
-                        else dst = (rb >= 1 && rb <= os) ? rb + nb - os - 1 : -1;
-                        if (dst >= s0 && dst <= hi) loc = dst - s0;
-                    }
-                    exact_add_chunk(loc, yv, iv, min(64, total - cb), my, mi,
-                                    st_y, st_i, cur, ay, ai);
-                }
-            }
-        } else {
         __syncthreads();
-        // the raw points of the next EXACT_AHEAD chunks are in flight while
-        // the wave walks the current one
+        // this wave's quarter of the bins
+        const int qn = (nb + EXW) / EXW;
+        const int qlo = wv * qn, qhi = min(qlo + qn, nb + 1) - 1;
         auto fetch = [&](int i, ExactPt &r) {
             if (i < n) {
                 r.t = b.t[off + i];
@@ -3124,188 +3147,260 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
                 r.d = R::dy(b, off + i);
             }
         };
-        ExactPt q[EXACT_AHEAD];
+        double ay = 0.0, ai = 0.0;
+        int cur = -1;
+        ExactPt qp[EXACT_AHEAD];
 #pragma unroll
-        for (int u = 0; u < EXACT_AHEAD; ++u) fetch(64 * u + lane, q[u]);
+        for (int u = 0; u < EXACT_AHEAD; ++u) fetch(64 * u + lane, qp[u]);
         for (int base0 = 0; base0 < n; base0 += 64 * EXACT_AHEAD) {
 #pragma unroll
-        for (int u = 0; u < EXACT_AHEAD; ++u) {
-            const int base = base0 + 64 * u;
-            if (base >= n) break;
-            const ExactPt r = q[u];
-            fetch(base + 64 * EXACT_AHEAD + lane, q[u]);
-            const double tp = r.t - P.t_ref;
-            const double iv = R::HAS_DY ? 1.0 / (r.d * r.d) : 1.0;
-            const double yv = (r.y - P.y_med) * iv;
-            const int ind = min(max(fold_bin(tp, period, invP, bd, inv_bd), 0),
-                                nb);
-            const int cnt = min(64, n - base);
-            if (fits) {
-                exact_add_chunk(ind, yv, iv, cnt, my, mi, st_y, st_i, cur,
-                                ay, ai);
-                continue;
-            }
-            // the wave walks the chunk in point order; every lane holds the
-            // same running values, lane 0 stores them
-            for (int j = 0; j < cnt; ++j) {
-                const double yvj = readlane_d(yv, j), ivj = readlane_d(iv, j);
-                const int indj = __builtin_amdgcn_readlane(ind, j);
-                int j1, j2;
-                pad_targets(indj, nb, os, j1, j2);
-                if (j1 >= s0 && j1 <= hi) {
-                    const double a = my[j1 - s0] + yvj;
-                    const double bb = mi[j1 - s0] + ivj;
-                    if (lane == 0) {
-                        my[j1 - s0] = a;
-                        mi[j1 - s0] = bb;
-                    }
-                }
-                if (j2 >= s0 && j2 <= hi) {
-                    const double a = my[j2 - s0] + yvj;
-                    const double bb = mi[j2 - s0] + ivj;
-                    if (lane == 0) {
-                        my[j2 - s0] = a;
-                        mi[j2 - s0] = bb;
-                    }
-                }
+            for (int u = 0; u < EXACT_AHEAD; ++u) {
+                const int base = base0 + 64 * u;
+                if (base >= n) break;
+                const ExactPt r = qp[u];
+                fetch(base + 64 * EXACT_AHEAD + lane, qp[u]);
+                const double tp = r.t - P.t_ref;
+                const double iv = R::HAS_DY ? 1.0 / (r.d * r.d) : 1.0;
+                const double yv = (r.y - P.y_med) * iv;
+                const int ind =
+                    min(max(fold_bin(tp, period, invP, bd, inv_bd), 0), nb);
+                const int loc = (ind >= qlo && ind <= qhi) ? ind : -1;
+                exact_add_chunk(loc, yv, iv, min(64, n - base), my, mi,
+                                st_y[wv], st_i[wv], cur, ay, ai);
             }
         }
-        }
+        if (cur >= 0 && lane == 0) {
+            my[cur] = ay;
+            mi[cur] = ai;
         }
         __syncthreads();
-        if (lane == 0) {
-            if ((fits || ranged) && cur >= 0) {
-                my[cur] = ay;
-                mi[cur] = ai;
+        if (tid == 0) {
+            for (int j = 1, d = nb - os; j <= os; ++j, ++d) {
+                my[d] = my[j];
+                mi[d] = mi[j];
             }
-            if (fits) {
-                for (int j = 1, d = nb - os; j <= os; ++j, ++d) {
-                    my[d] = my[j];
-                    mi[d] = mi[j];
+            exact_prefix(my, mi, nb + 1, true, 0.0, 0.0);
+        }
+        __syncthreads();
+        scan_seg(my, mi, 0, tid, nb, 64 * EXW);
+    } else if (ranged) {
+        const int nseg = (nb + S - 1) / S;
+        // the range searches of all segments in one round (all threads)
+        // when their table fits
+        const int main_e = 2 * ncyc * nseg;
+        const bool tab_ok = main_e + 2 * ncyc <= EXACT_TAB;
+        if (tab_ok) {
+            for (int e = tid; e < main_e + 2 * ncyc; e += 64 * EXW) {
+                int v = 0;
+                if (e < main_e) {
+                    const int j = e / (2 * ncyc), r = e - j * (2 * ncyc);
+                    const int s0j = j * S, hij = min(s0j + S + mmax, nb);
+                    v = lower_key(tn, n, period, invP, bd, inv_bd, nb,
+                                  (double)(r >> 1), (r & 1) ? hij + 1 : s0j);
+                } else {
+                    const int r = e - main_e;
+                    v = lower_key(tn, n, period, invP, bd, inv_bd, nb,
+                                  (double)(r >> 1), (r & 1) ? os + 1 : 1);
                 }
-            }
-            double cy = s0 == 0 ? my[0] : cprev_y + my[0];
-            double ci = s0 == 0 ? mi[0] : cprev_i + mi[0];
-            my[0] = cy;
-            mi[0] = ci;
-            // bins read eight at a time, one group ahead of the adds of the
-            // other (the adds stay in order; reads past the end are clamped
-            // and unused; the scheduling barriers keep the groups apart)
-            int l = 1;
-            double ay8[8], ai8[8], by8[8], bi8[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                ay8[u] = my[min(l + u, len - 1)];
-                ai8[u] = mi[min(l + u, len - 1)];
-            }
-            for (; l + 16 <= len; l += 16) {
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    by8[u] = my[l + 8 + u];
-                    bi8[u] = mi[l + 8 + u];
-                }
-                __builtin_amdgcn_sched_barrier(0);
-                // the adds of a group first, then its stores (a store waits
-                // for its add, and the wave issues in order)
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    cy = cy + ay8[u];
-                    ci = ci + ai8[u];
-                    ay8[u] = cy;
-                    ai8[u] = ci;
-                }
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    my[l + u] = ay8[u];
-                    mi[l + u] = ai8[u];
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    ay8[u] = my[min(l + 16 + u, len - 1)];
-                    ai8[u] = mi[min(l + 16 + u, len - 1)];
-                }
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    cy = cy + by8[u];
-                    ci = ci + bi8[u];
-                    by8[u] = cy;
-                    bi8[u] = ci;
-                }
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    my[l + 8 + u] = by8[u];
-                    mi[l + 8 + u] = bi8[u];
-                }
-                __builtin_amdgcn_sched_barrier(0);
-            }
-            if (l + 8 <= len) {
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    cy = cy + ay8[u];
-                    ci = ci + ai8[u];
-                    my[l + u] = cy;
-                    mi[l + u] = ci;
-                }
-                l += 8;
-            }
-            for (; l < len; ++l) {
-                cy = cy + my[l];
-                ci = ci + mi[l];
-                my[l] = cy;
-                mi[l] = ci;
-            }
-            if (!fits && s0 + S < nb) {
-                cprev_y = my[S - 1];
-                cprev_i = mi[S - 1];
+                s_tab[e] = v;
             }
         }
         __syncthreads();
-        const double ty = s_tot[0], ti = s_tot[1];
-        const int nend = min(s0 + S, nb);
-        // the oracle's formulas; a window of negative depth never wins, so
-        // its objective is not formed, and each objective forms only its
-        // own terms (the depth error and snr of the best window follow)
-        auto scan = [&](auto lk) {
-            constexpr bool LK = decltype(lk)::value;
-            for (int st = s0 + lane; st < nend; st += 64) {
-                const int l = st - s0;
-                for (int k = 0; k < nd; ++k) {
-                    const int m = s_m[k];
-                    if (st + m > nb) continue;
-                    const double ys = my[l + m] - my[l];
-                    const double is = mi[l + m] - mi[l];
-                    double y_in = ys, iv_in = is;
-                    double y_out = ty - y_in;
-                    double iv_out = ti - iv_in;
-                    if (iv_in < DBL_EPSILON || iv_out < DBL_EPSILON) continue;
-                    y_in /= iv_in;
-                    y_out /= iv_out;
-                    const double depth = y_out - y_in;
-                    if (!(depth >= 0.0)) continue;
-                    double obj;
-                    if (LK) {
-                        obj = 0.5 * iv_in * depth * depth;
-                    } else {
-                        const double derr = sqrt(1.0 / iv_in + 1.0 / iv_out);
-                        obj = depth / derr;
+        double cpy = 0.0, cpi = 0.0;  // thread 0: the carried prefix
+        for (int r0 = 0; r0 < nseg; r0 += EXW) {
+            const int seg = r0 + wv, s0 = seg * S;
+            const bool act = seg < nseg;
+            double *myw = my + wv * rb, *miw = mi + wv * rb;
+            if (act) {
+                const int hi = min(s0 + S + mmax, nb);
+                const int len = hi - s0 + 1;
+                int *lo_t = (int *)(myw + (rb - T));  // [2*ncyc] range starts
+                int *pre_t = lo_t + 2 * ncyc;         // [2*ncyc+1] prefix
+                for (int i = lane; i < len; i += 64) {
+                    myw[i] = 0.0;
+                    miw[i] = 0.0;
+                }
+                const bool extra = hi >= nb - os;
+                for (int e = lane; e < 4 * ncyc; e += 64) {
+                    const int k = e >> 1, c = k % ncyc;
+                    int v = 0;
+                    if (k < ncyc)
+                        v = tab_ok ? s_tab[2 * ncyc * seg + 2 * c + (e & 1)]
+                                   : lower_key(tn, n, period, invP, bd, inv_bd,
+                                               nb, (double)c,
+                                               (e & 1) ? hi + 1 : s0);
+                    else if (extra)
+                        v = tab_ok ? s_tab[main_e + 2 * c + (e & 1)]
+                                   : lower_key(tn, n, period, invP, bd, inv_bd,
+                                               nb, (double)c,
+                                               (e & 1) ? os + 1 : 1);
+                    if (e & 1) pre_t[k] = v;  // range end, until the prefix
+                    else lo_t[k] = v;
+                }
+                wave_lds_sync();
+                if (lane == 0) {
+                    int acc = 0;
+                    for (int k = 0; k < 2 * ncyc; ++k) {
+                        const int l = pre_t[k] - lo_t[k];
+                        pre_t[k] = acc;
+                        acc += l;
                     }
-                    const int idx = k * nval + st;
-                    if (better(obj, idx, bo, bi)) {
-                        bo = obj;
-                        bi = idx;
-                        byin = ys;
-                        bivin = is;
+                    pre_t[2 * ncyc] = acc;
+                }
+                wave_lds_sync();
+                const int total = pre_t[2 * ncyc];
+                // item -> (range, point): a lane's items rise by 64 from one
+                // chunk to the next, so its range cursor only moves forward;
+                // the raw points of the next EXACT_AHEAD chunks are in flight
+                // while the wave walks the current one
+                int kc = 0;
+                auto fetch = [&](int item, ExactPt &r) {
+                    if (item < total) {
+                        while (pre_t[kc + 1] <= item) ++kc;
+                        const int j = lo_t[kc] + (item - pre_t[kc]);
+                        r.k = kc;
+                        r.t = b.t[off + j];
+                        r.y = R::y(b, off + j);
+                        r.d = R::dy(b, off + j);
+                    }
+                };
+                double ay = 0.0, ai = 0.0;
+                int cur = -1;
+                ExactPt qp[EXACT_AHEAD];
+#pragma unroll
+                for (int u = 0; u < EXACT_AHEAD; ++u)
+                    fetch(64 * u + lane, qp[u]);
+                for (int cb0 = 0; cb0 < total; cb0 += 64 * EXACT_AHEAD) {
+#pragma unroll
+                    for (int u = 0; u < EXACT_AHEAD; ++u) {
+                        const int cb = cb0 + 64 * u;
+                        if (cb >= total) break;
+                        const ExactPt r = qp[u];
+                        fetch(cb + 64 * EXACT_AHEAD + lane, qp[u]);
+                        const int k = r.k;
+                        const bool valid = cb + lane < total;
+                        const double tp = r.t - P.t_ref;
+                        const double iv =
+                            R::HAS_DY ? 1.0 / (r.d * r.d) : 1.0;
+                        const double yv = (r.y - P.y_med) * iv;
+                        int loc = -1;
+                        if (valid) {
+                            const int c = k < ncyc ? k : k - ncyc;
+                            const int rbn = fold_bin_in(tp, period, (double)c,
+                                                        bd, inv_bd, nb);
+                            int dst;
+                            if (k < ncyc)
+                                dst = (rbn < nb - os || rbn == nb) ? rbn : -1;
+                            else
+                                dst = (rbn >= 1 && rbn <= os)
+                                          ? rbn + nb - os - 1
+                                          : -1;
+                            if (dst >= s0 && dst <= hi) loc = dst - s0;
+                        }
+                        exact_add_chunk(loc, yv, iv, min(64, total - cb), myw,
+                                        miw, st_y[wv], st_i[wv], cur, ay, ai);
+                    }
+                }
+                if (cur >= 0 && lane == 0) {
+                    myw[cur] = ay;
+                    miw[cur] = ai;
+                }
+            }
+            __syncthreads();
+            if (tid == 0) {
+                for (int ww = 0; ww < EXW && r0 + ww < nseg; ++ww) {
+                    const int sg = r0 + ww, s0j = sg * S;
+                    const int hij = min(s0j + S + mmax, nb);
+                    double *yw = my + ww * rb, *vw = mi + ww * rb;
+                    exact_prefix(yw, vw, hij - s0j + 1, sg == 0, cpy, cpi);
+                    if (s0j + S < nb) {
+                        cpy = yw[S - 1];
+                        cpi = vw[S - 1];
                     }
                 }
             }
-        };
-        if (like) scan(BoolC<true>());
-        else scan(BoolC<false>());
-        __syncthreads();
+            __syncthreads();
+            if (act) scan_seg(myw, miw, s0, s0 + lane, min(s0 + S, nb), 64);
+            __syncthreads();
+        }
+    } else {
+        double cpy = 0.0, cpi = 0.0;  // thread 0: the carried prefix
+        for (int s0 = 0; s0 < nb; s0 += S) {
+            const int hi = min(s0 + S + mmax, nb);
+            const int len = hi - s0 + 1;
+            for (int i = tid; i < len; i += 64 * EXW) {
+                my[i] = 0.0;
+                mi[i] = 0.0;
+            }
+            __syncthreads();
+            if (wv == 0) {
+                auto fetch = [&](int i, ExactPt &r) {
+                    if (i < n) {
+                        r.t = b.t[off + i];
+                        r.y = R::y(b, off + i);
+                        r.d = R::dy(b, off + i);
+                    }
+                };
+                ExactPt qp[EXACT_AHEAD];
+#pragma unroll
+                for (int u = 0; u < EXACT_AHEAD; ++u)
+                    fetch(64 * u + lane, qp[u]);
+                for (int base0 = 0; base0 < n; base0 += 64 * EXACT_AHEAD) {
+#pragma unroll
+                    for (int u = 0; u < EXACT_AHEAD; ++u) {
+                        const int base = base0 + 64 * u;
+                        if (base >= n) break;
+                        const ExactPt r = qp[u];
+                        fetch(base + 64 * EXACT_AHEAD + lane, qp[u]);
+                        const double tp = r.t - P.t_ref;
+                        const double iv =
+                            R::HAS_DY ? 1.0 / (r.d * r.d) : 1.0;
+                        const double yv = (r.y - P.y_med) * iv;
+                        const int ind = min(
+                            max(fold_bin(tp, period, invP, bd, inv_bd), 0), nb);
+                        const int cnt = min(64, n - base);
+                        // the wave walks the chunk in point order; every lane
+                        // holds the same running values, lane 0 stores them
+                        for (int j = 0; j < cnt; ++j) {
+                            const double yvj = readlane_d(yv, j),
+                                         ivj = readlane_d(iv, j);
+                            const int indj = __builtin_amdgcn_readlane(ind, j);
+                            int j1, j2;
+                            pad_targets(indj, nb, os, j1, j2);
+                            if (j1 >= s0 && j1 <= hi) {
+                                const double a = my[j1 - s0] + yvj;
+                                const double bb = mi[j1 - s0] + ivj;
+                                if (lane == 0) {
+                                    my[j1 - s0] = a;
+                                    mi[j1 - s0] = bb;
+                                }
+                            }
+                            if (j2 >= s0 && j2 <= hi) {
+                                const double a = my[j2 - s0] + yvj;
+                                const double bb = mi[j2 - s0] + ivj;
+                                if (lane == 0) {
+                                    my[j2 - s0] = a;
+                                    mi[j2 - s0] = bb;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            if (tid == 0) {
+                exact_prefix(my, mi, len, s0 == 0, cpy, cpi);
+                if (s0 + S < nb) {
+                    cpy = my[S - 1];
+                    cpi = mi[S - 1];
+                }
+            }
+            __syncthreads();
+            scan_seg(my, mi, s0, s0 + tid, min(s0 + S, nb), 64 * EXW);
+            __syncthreads();
+        }
     }
+    // the best window over the workgroup
     for (int ofs = 32; ofs > 0; ofs >>= 1) {
         const double ob = __shfl_xor(bo, ofs);
         const int oi = __shfl_xor(bi, ofs);
@@ -3317,8 +3412,24 @@ __global__ __launch_bounds__(64) void k_exact(Batch b, Out o, Work w,
             bivin = ov;
         }
     }
-    if (lane == 0 && bi != INT_MAX) {
-        const double ty = s_tot[0], ti = s_tot[1];
+    if (lane == 0) {
+        r_bo[wv] = bo;
+        r_bi[wv] = bi;
+        r_y[wv] = byin;
+        r_i[wv] = bivin;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int k = 1; k < EXW; ++k) {
+            if (better(r_bo[k], r_bi[k], bo, bi)) {
+                bo = r_bo[k];
+                bi = r_bi[k];
+                byin = r_y[k];
+                bivin = r_i[k];
+            }
+        }
+    }
+    if (tid == 0 && bi != INT_MAX) {
         const int k = bi / nval, st = bi - k * nval, m = s_m[k];
         double y_in = byin, iv_in = bivin;
         double y_out = ty - y_in;
@@ -4266,11 +4377,12 @@ int run_impl(atg_ctx *c, const atg_bls_batch *b, const atg_bls_out *o,
         c->pending.emplace_back(ev0, ev1);
     }
     hipLaunchKernelGGL(k_cands, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w);
-    const int ex_bins = std::min(c->tune.lds_bins, EXACT_BINS);
+    const int ex_total = exact_total(c->tune.lds_bins);
     with_raw_types(kb, [&](auto r) {
         hipLaunchKernelGGL(k_exact<decltype(r)>,
-                           dim3(c->num_cu * EXACT_WG_PER_CU), dim3(64),
-                           (size_t)ex_bins * 16, c->stream, kb, ko, w, ex_bins);
+                           dim3(c->num_cu * EXACT_WG_PER_CU), dim3(64 * EXW),
+                           (size_t)16 * (ex_total + 8), c->stream, kb, ko, w,
+                           ex_total / EXW);
     });
     hipLaunchKernelGGL(k_final, dim3(nlc), dim3(NT), 0, c->stream, kb, ko, w,
                        kbest);
