@@ -2226,6 +2226,9 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
             cw = block_reduce(cw, AddOp<long long>(), lscr);
             cy = block_reduce(cy, AddOp<long long>(), lscr);
         }
+        // the segment's prefix and box scan above the normal priority, as
+        // k_main's finish (long-baseline stage -1%)
+        __builtin_amdgcn_s_setprio(1);
         const int seg = (len + NT - 1) / NT;
         const int b0 = min(tid * seg, len), b1 = min(b0 + seg, len);
         long long sw = 0, sy = 0;
@@ -2280,6 +2283,7 @@ __device__ void big_period(const Batch &b, const Out &o, const Work &w,
             }
         }
         __syncthreads();
+        __builtin_amdgcn_s_setprio(0);
     }
     for (int ofs = 32; ofs > 0; ofs >>= 1) {
         const double oq = __shfl_xor(best.q, ofs), od = __shfl_xor(best.den, ofs);
@@ -2531,6 +2535,11 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
                 ++p;
                 continue;
             }
+            // the pass's binning at the normal priority, its finish above it
+            // (latency-bound chains of LDS reads: ready finish instructions
+            // issue ahead of the other workgroups' binning on the SIMD;
+            // stage 48.1 -> 47.4 ms at the headline shape)
+            __builtin_amdgcn_s_setprio(0);
             __syncthreads();  // the previous pass's last reads
             lds_zero(binW, used * 8);
             if (counted) lds_zero(cntb, used * 4);
@@ -2569,6 +2578,7 @@ void k_main(Batch b, Out o, Work w, int per_item, int lds_bins) {
 #undef ATG_BIN
             }
             __syncthreads();
+            __builtin_amdgcn_s_setprio(1);
 
             if (wmode) {
                 // wave w finishes the pass's periods w, w + 4, ...
