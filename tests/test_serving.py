@@ -108,3 +108,30 @@ def test_served_large_rounds_equal_alone():
         for k in h.result:
             assert np.array_equal(h.result[k], alone[k]), (i, k)
         assert h.best_index == int(alone["best_index"][0])
+
+
+@pytest.mark.gpu
+def test_served_mixed_weighting_equals_alone():
+    """Requests with and without per-point dy, interleaved: batches split
+    where the weighting changes and the dy ring holds only the requests
+    that have one; every result equals its alone run bit for bit."""
+    from astro_transit_gpu_amd import Context, DeviceBatch
+    from astro_transit_gpu_amd.serving import BLSServer
+    reqs = _requests(12, 41)
+    reqs = [(t, y, None if i % 3 == 1 else dy, per, durs)
+            for i, (t, y, dy, per, durs) in enumerate(reqs)]
+    srv = BLSServer(objective="likelihood", capacity_points=200000,
+                    capacity_periods=20000, capacity_durations=200,
+                    max_batch=4)
+    handles = [srv.submit(*r) for r in reqs]
+    srv.drain()
+    ctx = Context(0)
+    for i, (h, r) in enumerate(zip(handles, reqs)):
+        assert h.status == 0
+        b = DeviceBatch.from_host([r[0]], [r[1]],
+                                  None if r[2] is None else [r[2]], r[3],
+                                  r[4], device=ctx.device)
+        alone = ctx.run(b, objective="likelihood").to_host()
+        for k in h.result:
+            assert np.array_equal(h.result[k], alone[k]), (i, k)
+        assert h.best_index == int(alone["best_index"][0])
