@@ -397,6 +397,18 @@ class BLSServer:
                 if self.inflight:
                     self.poll(block=True)
 
+    def close(self):
+        """Stop the host staging threads (the server can still be used;
+        they start again on the next large round)."""
+        if self._pool is not None:
+            self._pool.shutdown(wait=True)
+            self._pool = None
+
+    def __del__(self):
+        pool = getattr(self, "_pool", None)
+        if pool is not None:
+            pool.shutdown(wait=False)
+
     def run_alone(self, req_args):
         """Serve one request by itself (for equality checks)."""
         r = self.submit(*req_args)
