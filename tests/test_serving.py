@@ -99,6 +99,7 @@ def test_served_large_rounds_equal_alone():
                     capacity_durations=200)
     handles = [srv.submit(*r) for r in reqs]  # one round of ~40 MB
     srv.drain()
+    srv.close()
     ctx = Context(0)
     for i, (h, r) in enumerate(zip(handles, reqs)):
         assert h.status == 0
