@@ -481,16 +481,27 @@ __device__ __forceinline__ void bin_points(
     double tv_n = load_at<double>(bt, lo);
     long long wv_n = UW ? 1 : load_at<long long>(bw, lo);
     long long yv_n = PACKED ? 0 : load_at<long long>(by, lo);
-    const double tol2 = 2.0 * tol_x;
-    // per-period origins at bin 1, so that bin at + 1's LDS address is one
-    // shift-and-add of at
+    // Exact fold: the shifted quotient is computed as XB + x (XB = 2^20,
+    // above every bin of the histogram), so that its exponent is fixed and
+    // its fraction is exactly the low 32-bit word of the mantissa (units of
+    // 2^-32): a fraction of at most 2*tolk is a word of at most tq, one
+    // unsigned compare.  tolk adds 2^-29 to the curve's tolerance for the
+    // roundings the shift adds (at most 4*XB*2^-53 = 2^-31).
+    constexpr double XB = 1048576.0;
+    constexpr int XBI = 1 << 20;
+    const double tolk = tol_x + 0x1p-29, tbc = XB + tolk;
+    const unsigned tq = (unsigned)(2.0 * tolk * 0x1p32) + 1u;
+    // per-period origins at bin 1 - XB, so that bin at - XB + 1's LDS
+    // address is one shift-and-add of at (32-bit LDS addresses: the
+    // origin wraps, the sum does not)
     long long *ow[C], *oy[C];
     int *oc[C];
 #pragma unroll
     for (int s = 0; s < C; ++s) {
-        ow[s] = binW + base[s] + 1;
-        oy[s] = binY + base[s] + 1;
-        oc[s] = (int *)binW + base[s] + 1;
+        const int o = base[s] + 1 - (EXACT ? XBI : 0);
+        ow[s] = binW + o;
+        oy[s] = binY + o;
+        oc[s] = (int *)binW + o;
     }
     // two points per iteration: the prefetched registers alternate instead
     // of being copied
@@ -505,37 +516,37 @@ __device__ __forceinline__ void bin_points(
             if (!PACKED) yv_n = load_at<long long>(by, lo);
         }
         int at[C];
-        // any period's quotient near an integer; the flags themselves are
-        // recomputed on the rare path below rather than held through the
-        // fold (C lane masks of scalar registers, which then spill)
-        bool any = false;
-        // t'/bd + tol_x, shared by the C periods of the pass (one rounding;
-        // within the tolerance analysis below, which allows two)
-        const double tb = EXACT ? fma(tv, inv_bd, tol_x) : 0.0;
+        // the least low word of the C shifted quotients: some period's
+        // quotient is near an integer if it is <= tq; the flags themselves
+        // are recomputed on the rare path below rather than held through
+        // the fold (C lane masks of scalar registers, which then spill)
+        unsigned mlo = 0xffffffffu;
+        // XB + t'/bd + tolk, shared by the C periods of the pass
+        const double tb = EXACT ? fma(tv, inv_bd, tbc) : 0.0;
 #pragma unroll
         for (int s = 0; s < C; ++s) {
             if (EXACT) {
                 // bin from the rounded reciprocals; a quotient within the
                 // uniform tolerance of an integer is flagged for the
                 // oracle's divisions.  Unflagged bins lie in [1, nb - os].
-                // per[s] holds P/bd here: x = t'/bd - q*P/bd + tol_x is
-                // within u*(7*t'/bd + 6*P/bd) of the oracle's
-                // (t' - P*q)/bd + tol_x (u = 2^-53: the roundings of the
-                // products, the shift and the FMA against the oracle's
-                // three), less than tol_x.  So an x whose fraction exceeds
-                // 2*tol_x lies in the oracle's integer interval, and the
-                // conversion's truncation is the oracle's floor; a fraction
-                // up to 2*tol_x is flagged.  inv[s] is 1/P scaled up by
-                // 1 + 2^-48, so t' * inv[s] exceeds the oracle's rounded
-                // t'/P and the cycle q is the oracle's or one too high; one
-                // too high (t'/P within ~36 ulp below an integer) puts the
-                // unshifted quotient within u*(40*t'/bd + 39*P/bd) below 0,
-                // so x within [0, 2*tol_x]: flagged.
+                // per[s] holds P/bd here: x = XB + t'/bd - q*P/bd + tolk
+                // is within u*(7*t'/bd + 6*P/bd) + 4*XB*u of the oracle's
+                // XB + (t' - P*q)/bd + tolk (u = 2^-53: the roundings of
+                // the products, the shift and the FMA against the oracle's
+                // three, and the shift's own), less than tolk.  So an x
+                // whose fraction exceeds 2*tolk lies in the oracle's
+                // integer interval, and the conversion's truncation is the
+                // oracle's floor; a fraction up to 2*tolk is flagged.
+                // inv[s] is 1/P scaled up by 1 + 2^-48, so t' * inv[s]
+                // exceeds the oracle's rounded t'/P and the cycle q is the
+                // oracle's or one too high; one too high (t'/P within ~36
+                // ulp below an integer) puts the unshifted quotient within
+                // u*(40*t'/bd + 39*P/bd) below 0, so x within
+                // [XB, XB + 2*tolk]: flagged.
                 const double q = floor(tv * inv[s]);
                 const double x = fma(q, -per[s], tb);
-                const double fx = __builtin_amdgcn_fract(x);
-                any |= fx <= tol2;
-                at[s] = (int)x;  // bin at[s] + 1
+                mlo = min(mlo, (unsigned)__double2loint(x));
+                at[s] = (int)x;  // bin at[s] - XB + 1
             } else {
                 // fast mode: FMA phase and rounded reciprocals only; a point
                 // within ~1e-15 of a bin edge may land in the neighbour bin.
@@ -561,7 +572,7 @@ __device__ __forceinline__ void bin_points(
                 __hip_atomic_fetch_add(oy[s] + at[s], yv, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
         }
-        if (EXACT && any) {
+        if (EXACT && mlo <= tq) {
             // Every point went to its reciprocal bin above, so the atomics
             // share the fold's basic block.  A flagged point whose oracle
             // bin (both divisions) differs moves there: integer adds
@@ -575,17 +586,26 @@ __device__ __forceinline__ void bin_points(
             // registers through the loop (pg is opaque to hoisting)
             const double *pgx = pg;
             asm volatile("" : "+s"(pgx));
+            // the oracle's bin of a point in its first cycle, the same for
+            // every period of the pass: t' * inv[s] < 0.9375 gives
+            // floor(t'/P) = 0 and the phase t' itself.  On a regular
+            // cadence about half of the flags come from such points (t'/bd
+            // an exact integer).
+            const int e0 = (int)trunc(tx / bd) + 1;
 #pragma unroll
             for (int s = 0; s < C; ++s) {
                 // the main path's fold again, and its flag
                 const double xs =
-                    fma(floor(tx * inv[s]), -per[s], fma(tx, inv_bd, tol_x));
-                if (__builtin_amdgcn_fract(xs) <= tol2) {
-                    const double Ps = pgx[s];
-                    const int a = base[s] + 1 + (int)xs;
-                    const double ph = tx - Ps * floor(tx / Ps);
-                    const int e =
-                        base[s] + min(max((int)trunc(ph / bd) + 1, 0), nb[s]);
+                    fma(floor(tx * inv[s]), -per[s], fma(tx, inv_bd, tbc));
+                if ((unsigned)__double2loint(xs) <= tq) {
+                    const int a = base[s] + 1 + ((int)xs - XBI);
+                    int e0s = e0;
+                    if (!(tx * inv[s] < 0.9375)) {
+                        const double Ps = pgx[s];
+                        const double ph = tx - Ps * floor(tx / Ps);
+                        e0s = (int)trunc(ph / bd) + 1;
+                    }
+                    const int e = base[s] + min(max(e0s, 0), nb[s]);
                     if (e != a) {
                         if (UW) {
                             __hip_atomic_fetch_add(&((int *)binW)[a], -1,
