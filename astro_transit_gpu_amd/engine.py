@@ -33,6 +33,25 @@ def _dev(device):
     return d
 
 
+def release_host_cache():
+    """Returns the pinned host blocks torch keeps cached (freed staging
+    buffers) to the system."""
+    torch._C._host_emptyCache()
+
+
+def pinned_empty(n, dtype):
+    """A pinned host tensor of n elements.  Freed pinned buffers stay in
+    torch's host cache; when a new one does not fit beside them, the cache
+    is released and the allocation tried once more."""
+    try:
+        return torch.empty(n, dtype=dtype, pin_memory=True)
+    except RuntimeError as e:
+        if "out of memory" not in str(e):
+            raise
+        release_host_cache()
+        return torch.empty(n, dtype=dtype, pin_memory=True)
+
+
 @dataclass
 class DeviceBatch:
     """CSR batch: curve i owns points lc_offsets[i]:lc_offsets[i+1], periods
@@ -110,8 +129,7 @@ class DeviceBatch:
             offs.append(total)
             if a is not None:
                 total += (a.nbytes + 255) // 256 * 256
-        host = torch.empty(max(total, 256), dtype=torch.uint8,
-                           pin_memory=True)
+        host = pinned_empty(max(total, 256), torch.uint8)
         hv = host.numpy()
         for a, o in zip(arrays, offs):
             if a is not None and a.nbytes:

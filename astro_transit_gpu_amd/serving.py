@@ -29,7 +29,7 @@ from typing import List, Optional
 import numpy as np
 import torch
 
-from .engine import FIELDS, Context, DeviceBatch, DeviceResult
+from .engine import FIELDS, Context, DeviceBatch, DeviceResult, pinned_empty
 
 def _al(n):
     # requests sit back to back: a batch's CSR ranges must be contiguous
@@ -248,7 +248,7 @@ class BLSServer:
         for _, start, arrs, dt in runs:
             offs.append(total)
             total += (sum(a.size for a in arrs) * dt.itemsize + 63) & ~63
-        pin = torch.empty(total, dtype=torch.uint8, pin_memory=True)
+        pin = pinned_empty(total, torch.uint8)
         host = pin.numpy()
         pieces = []  # (output view, arrays): contiguous sub-ranges of a run
         for (_, _, arrs, dt), off in zip(runs, offs):

@@ -739,7 +739,12 @@ def main():
                                              objective=objective)
         except ImportError:
             line["served"] = None
+        from astro_transit_gpu_amd.engine import release_host_cache
+        # the server's freed staging buffers stay in torch's pinned cache;
+        # the next legs stage gigabytes of their own
+        release_host_cache()
         line["ragged"] = ragged_leg(args.ragged_lc, objective)
+        release_host_cache()
         line["served_stream"] = serve5_leg(args.serve5_seconds, objective)
         ctx.run(batch, objective=objective, mode="parity", out=out)
         line["cpu_baseline"], line["parity_vs_oracle"] = cpu_baseline(
